@@ -1,0 +1,91 @@
+// coherence.hip -- boxcar interferometric coherence (gfx950).
+//
+// Semantics of EstCoh.cu:101-139 (CohMatSet, short2 master) and 413-456
+// (Cohbasic): per output (r, c) the window covers rows r-dy .. r-dy+winAz-1
+// and columns c-dx .. c-dx+winRg-1 of the burst, zero outside it (the
+// reference leaves that padding uninitialised, EstCoh.cu:581-583);
+//   coh = |sum M conj(S)| / sqrt(sum|S|^2 * sum|M|^2)  (0 when the product
+//   is not positive), and coh = 0 where |S(r, c)| < 0.001.
+//
+// Fast form: a 16x64 output tile stages the products (M conj S, |S|^2,
+// |M|^2) of its (16+winAz-1) x (64+winRg-1) window in LDS, forms
+// compensated (Kahan) FP32 range sums, then short azimuth sums.
+// FP64 form: one thread per output, direct FP64 window sums from HBM.
+#include "tops_internal.hpp"
+#include "tile_window.hpp"
+
+namespace tops {
+
+__global__ void __launch_bounds__(256) coherence_fp64_kernel(const tops_ci16* M, const tops_cf32* S, int w,
+                                      int h, int dx, int dy, int win_rg, int win_az,
+                                      float* coh)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (c >= w) return;
+    size_t q0 = (size_t)r * w + c;
+    tops_cf32 sc = S[q0];
+    double mag = sqrt((double)sc.x * sc.x + (double)sc.y * sc.y);
+    if (mag < 0.001) { coh[q0] = 0.0f; return; }
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int j = 0; j < win_az; j++) {
+        int rr = r - dy + j;
+        if (rr < 0 || rr >= h) continue;
+        for (int k = 0; k < win_rg; k++) {
+            int cc = c - dx + k;
+            if (cc < 0 || cc >= w) continue;
+            size_t q = (size_t)rr * w + cc;
+            tops_ci16 m = M[q];
+            tops_cf32 s = S[q];
+            double mr = m.re, mi = m.im, sr = s.x, si = s.y;
+            a0 += mr * sr + mi * si;
+            a1 += mi * sr - mr * si;
+            a2 += sr * sr + si * si;
+            a3 += mr * mr + mi * mi;
+        }
+    }
+    double pp = a2 * a3;
+    coh[q0] = pp > 0.0 ? (float)(sqrt(a0 * a0 + a1 * a1) / sqrt(pp)) : 0.0f;
+}
+
+__global__ void __launch_bounds__(256) coherence_fast_kernel(const tops_ci16* M, const tops_cf32* S, int w,
+                                      int h, int dx, int dy, int win_rg, int win_az,
+                                      float* coh)
+{
+    extern __shared__ float4 smem[];
+    const int r0 = blockIdx.y * TW_ROWS, c0 = blockIdx.x * TW_COLS;
+    WindowSums ws[4];
+    tile_window_sums(M, S, w, h, r0, c0, dx, dy, win_rg, win_az, smem, ws, nullptr);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = c0 + lane;
+    if (c >= w) return;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int r = r0 + wv * 4 + k;
+        if (r >= h) break;
+        size_t q = (size_t)r * w + c;
+        tops_cf32 s = S[q];
+        float out = 0.0f;
+        if (sqrtf(s.x * s.x + s.y * s.y) >= 0.001f) out = ws[k].coherence();
+        coh[q] = out;
+    }
+}
+
+int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h, int dx,
+                     int dy, int win_rg, int win_az, float* coh, int mode, hipStream_t st)
+{
+    if (mode == TOPS_MODE_FP64 || !tile_window_fits(win_rg, win_az)) {
+        dim3 g((w + 255) / 256, h);
+        hipLaunchKernelGGL(coherence_fp64_kernel, g, dim3(256), 0, st, m, s, w, h, dx, dy,
+                           win_rg, win_az, coh);
+    } else {
+        dim3 g((w + TW_COLS - 1) / TW_COLS, (h + TW_ROWS - 1) / TW_ROWS);
+        hipLaunchKernelGGL(coherence_fast_kernel, g, dim3(256),
+                           tile_window_lds_bytes(win_rg, win_az), st, m, s, w, h, dx, dy,
+                           win_rg, win_az, coh);
+    }
+    TOPS_HIP(hipGetLastError());
+    return TOPS_OK;
+}
+
+}  // namespace tops

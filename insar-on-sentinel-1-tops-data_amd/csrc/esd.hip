@@ -1,0 +1,174 @@
+// esd.hip -- enhanced spectral diversity over burst-overlap bands (gfx950).
+//
+// Semantics of EstESDShifts_NonOverlap (EstESD.cu:914-1217): for overlap i
+// with n_i lines, over the forward band (last n_i lines of burst i) and the
+// backward band (first n_i lines of burst i+1):
+//   coh_f, coh_b : boxcar coherence, window winAz x winRg starting
+//                  (winAz-1)/2 rows and (winRg-1)/2 columns before the
+//                  sample, zero outside the band, no amplitude mask
+//                  (CohMatrixSet, CohFirstLineSet_ESD, CohDiffer_vertical_ESD,
+//                  shfl_vertical_ESD, CpCoh2_ESD)
+//   coh          : (coh_f + coh_b) / 2                       (Divide2)
+//   dd           : (Mf conj Sf) conj(Mb conj Sb)              (DoubleDifference4OneD)
+//   phasor       : 0 if coh < thr or coh >= 1, else exp(j arg dd) (MaskOut)
+//   sum_i        : sum of phasors over the band               (reduce0 x3)
+// The shift itself, atan2(sum_i) / (2 pi df dt), is formed on the host.
+//
+// One fused kernel per call: each 16x64 tile computes both band coherences
+// from LDS (tile_window_sums), the double difference and the masked
+// phasors, sums them per thread with Kahan FP32, reduces the workgroup by
+// wave shuffles and writes one FP64 partial; a second kernel adds the
+// partials of each band in FP64 in a fixed order.
+#include "tops_internal.hpp"
+#include "tile_window.hpp"
+
+namespace tops {
+
+struct EsdBand {
+    int line_offset;   // first line of the band in the concatenated arrays
+    int lines;         // n_i
+};
+
+constexpr int kMaxEsdBands = 64;
+
+struct EsdParams {
+    const tops_ci16* mf;
+    const tops_ci16* mb;
+    const tops_cf32* sf;
+    const tops_cf32* sb;
+    int pixels, win_az, win_rg;
+    float thr;
+    int tiles_x, tiles_y;
+    int n_bands;
+    EsdBand band[kMaxEsdBands];
+    double2* partials;   // [n_bands][tiles_y][tiles_x]
+};
+
+__device__ inline float wave_sum(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) esd_band_kernel(EsdParams p)
+{
+    extern __shared__ float4 smem[];
+    __shared__ float red[4][2];
+    const int b = blockIdx.z;
+    const int n = p.band[b].lines;
+    const int r0 = blockIdx.y * TW_ROWS, c0 = blockIdx.x * TW_COLS;
+    double2* out = p.partials + ((size_t)b * p.tiles_y + blockIdx.y) * p.tiles_x + blockIdx.x;
+    if (r0 >= n) {   // whole workgroup leaves together: no barrier skipped
+        if (threadIdx.x == 0) *out = make_double2(0.0, 0.0);
+        return;
+    }
+    const size_t off = (size_t)p.band[b].line_offset * p.pixels;
+    const int dh = (p.win_az - 1) / 2, dw = (p.win_rg - 1) / 2;
+    WindowSums wf[4], wb[4];
+    float2 cf[4], cb[4];
+    tile_window_sums(p.mf + off, p.sf + off, p.pixels, n, r0, c0, dw, dh, p.win_rg,
+                     p.win_az, smem, wf, cf);
+    tile_window_sums(p.mb + off, p.sb + off, p.pixels, n, r0, c0, dw, dh, p.win_rg,
+                     p.win_az, smem, wb, cb);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float sx = 0.f, sy = 0.f, ex = 0.f, ey = 0.f;
+    if (c0 + lane < p.pixels) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            int r = r0 + wv * 4 + k;
+            if (r >= n) break;
+            float coh = (wf[k].coherence() + wb[k].coherence()) / 2.0f;
+            if (coh < p.thr || coh >= 1.0f) continue;
+            // exp(j arg(a conj b)) from the unit phasors of a and b
+            float na = sqrtf(cf[k].x * cf[k].x + cf[k].y * cf[k].y);
+            float nb = sqrtf(cb[k].x * cb[k].x + cb[k].y * cb[k].y);
+            float px = 1.0f, py = 0.0f;   // atan2(0, 0) = 0
+            if (na > 0.0f && nb > 0.0f) {
+                float ax = cf[k].x / na, ay = cf[k].y / na;
+                float bx = cb[k].x / nb, by = cb[k].y / nb;
+                float dx = ax * bx + ay * by, dy = ay * bx - ax * by;
+                float nd = sqrtf(dx * dx + dy * dy);
+                px = dx / nd;
+                py = dy / nd;
+            }
+            kahan_add(sx, ex, px);
+            kahan_add(sy, ey, py);
+        }
+    }
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    if (lane == 0) { red[wv][0] = sx; red[wv][1] = sy; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tx = 0.0, ty = 0.0;
+        for (int w = 0; w < 4; w++) { tx += red[w][0]; ty += red[w][1]; }
+        *out = make_double2(tx, ty);
+    }
+}
+
+__global__ void __launch_bounds__(256) esd_final_kernel(const double2* partials, int per_band, double* sums)
+{
+    __shared__ double sx[256], sy[256];
+    const double2* q = partials + (size_t)blockIdx.x * per_band;
+    double ax = 0.0, ay = 0.0;
+    for (int i = threadIdx.x; i < per_band; i += 256) { ax += q[i].x; ay += q[i].y; }
+    sx[threadIdx.x] = ax;
+    sy[threadIdx.x] = ay;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            sx[threadIdx.x] += sx[threadIdx.x + s];
+            sy[threadIdx.x] += sy[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sums[2 * blockIdx.x] = sx[0];
+        sums[2 * blockIdx.x + 1] = sy[0];
+    }
+}
+
+int esd_partial_capacity(const int* overlap_size, int n_overlap, int pixels)
+{
+    int maxn = 0;
+    for (int i = 0; i < n_overlap; i++) maxn = overlap_size[i] > maxn ? overlap_size[i] : maxn;
+    int tx = (pixels + TW_COLS - 1) / TW_COLS, ty = (maxn + TW_ROWS - 1) / TW_ROWS;
+    return n_overlap * tx * ty;
+}
+
+int launch_esd(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* sf,
+               const tops_cf32* sb, const int* overlap_size, int n_overlap, int pixels,
+               int win_az, int win_rg, float thr, double* d_sums, double* d_partials,
+               int /*max_blocks_per_band*/, hipStream_t st)
+{
+    if (n_overlap <= 0) return TOPS_OK;
+    if (n_overlap > kMaxEsdBands)
+        return set_error(TOPS_ERR_ARG, "EstESDShifts: at most %d overlaps per call", kMaxEsdBands);
+    if (!tile_window_fits(win_rg, win_az))
+        return set_error(TOPS_ERR_ARG, "EstESDShifts: window %dx%d exceeds the LDS tile", win_az,
+                         win_rg);
+    EsdParams p;
+    p.mf = mf; p.mb = mb; p.sf = sf; p.sb = sb;
+    p.pixels = pixels; p.win_az = win_az; p.win_rg = win_rg; p.thr = thr;
+    int maxn = 0, off = 0;
+    for (int i = 0; i < n_overlap; i++) {
+        p.band[i].line_offset = off;
+        p.band[i].lines = overlap_size[i];
+        off += overlap_size[i];
+        if (overlap_size[i] > maxn) maxn = overlap_size[i];
+    }
+    p.n_bands = n_overlap;
+    p.tiles_x = (pixels + TW_COLS - 1) / TW_COLS;
+    p.tiles_y = maxn > 0 ? (maxn + TW_ROWS - 1) / TW_ROWS : 1;
+    p.partials = (double2*)d_partials;
+    dim3 grid(p.tiles_x, p.tiles_y, n_overlap);
+    hipLaunchKernelGGL(esd_band_kernel, grid, dim3(256), tile_window_lds_bytes(win_rg, win_az),
+                       st, p);
+    TOPS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(esd_final_kernel, dim3(n_overlap), dim3(256), 0, st,
+                       (const double2*)d_partials, p.tiles_x * p.tiles_y, d_sums);
+    TOPS_HIP(hipGetLastError());
+    return TOPS_OK;
+}
+
+}  // namespace tops

@@ -1,0 +1,122 @@
+// tops_internal.hpp -- shared definitions of the MI355X TOPS burst library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "tops_abi.h"
+
+namespace tops {
+
+constexpr double kPi = 3.1415926535897931;        // Constants.h:8
+constexpr double kTwoPi = 6.283185307179586232;
+constexpr double kInvTwoPi = 0.15915494309189533577;
+constexpr double kSol = 299792458.0;              // cuConstants.cuh:7
+constexpr double kEllA = 6378137.0;               // cuConstants.cuh:5
+constexpr double kEllB = 6356752.3142451794975639665996337;  // cuConstants.cuh:6
+constexpr int kLutRows = 2048;                    // Interval + 1, Constants.h:9
+constexpr int kInterval = 2047;
+constexpr int kTaps = 12;                         // Main.cpp:75, 113
+constexpr int kHalf = kTaps / 2 - 1;              // c_Npoints2m1
+
+// ---- status (thread-local, read through tops_last_error) ---------------
+int set_error(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define TOPS_HIP(call)                                                       \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) return ::tops::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+// ---- resample context ---------------------------------------------------
+struct Scratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);   // grows with hipMalloc outside any capture
+    void release();
+};
+
+}  // namespace tops
+
+struct tops_ctx {
+    float* d_kaz = nullptr;     // [2048 x 12] azimuth LUT
+    float* d_krg = nullptr;     // [2048 x 12] range LUT
+    double* d_mom = nullptr;    // [2048 x 4] azimuth LUT moments M0 M1 M2 0
+    int npoints = 12;
+    tops::Scratch scratch;      // FP64 baseline intermediates
+};
+
+namespace tops {
+
+// 6-term polynomial, cuConstants.cuh:57-66.  Evaluated left to right with
+// FMA contraction off: the slave position selects a LUT row by truncation,
+// so it must be bit-identical wherever it is computed (positions on an
+// exact LUT-row boundary occur for offset fields with short decimal
+// coefficients).
+__host__ __device__ inline double polyval6(double x, double y, const double* c)
+{
+#pragma clang fp contract(off)
+    return c[0] + c[1] * x + c[2] * y + c[3] * x * x + c[4] * x * y + c[5] * y * y;
+}
+
+// cuConstants.cuh:52-55 (contraction off, as polyval6)
+__host__ __device__ inline double norm_warp(double v, double lo, double hi)
+{
+#pragma clang fp contract(off)
+    return (v - (0.5 * (lo + hi))) / (0.25 * (hi - lo));
+}
+
+// Per-call resample parameters passed by value to the kernels.
+struct ResampleParams {
+    int mb[4];          // master box
+    int sb[4];          // slave box
+    int m_lines, m_pixels, s_lines, s_pixels;
+    int first_line;     // first_line_in_burst
+    double dt;          // azimuth time interval
+    double az_shift;
+    double cpm_az[6];
+    double cpm_rg[6];
+    const tops_ci16* slave;
+    const double* kt;
+    const double* tref;
+    const double* fdc;
+    const float* kaz;
+    const float* krg;
+    const double* mom;
+    tops_cf32* out;
+};
+
+// launchers (resample.hip)
+int launch_resample_fp64(tops_ctx* ctx, const ResampleParams& p, hipStream_t s);
+int launch_resample_fast(const ResampleParams& p, hipStream_t s);
+
+// launchers (coherence.hip)
+int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h,
+                     int dx, int dy, int win_rg, int win_az, float* coh,
+                     int mode, hipStream_t st);
+
+// launchers (esd.hip): writes 2*n_overlap double sums to d_sums
+int launch_esd(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* sf,
+               const tops_cf32* sb, const int* overlap_size, int n_overlap,
+               int pixels, int win_az, int win_rg, float thr, double* d_sums,
+               double* d_partials, int max_blocks_per_band, hipStream_t st);
+int esd_partial_capacity(const int* overlap_size, int n_overlap, int pixels);
+
+// launchers (geocoreg.hip)
+struct OrbitDev {
+    double cx[6], cy[6], cz[6];
+    int ncoef;
+    double mid_time, dt, first_line_time, slr_first, range_spacing, az_init;
+    int line_offset;
+};
+int launch_geocoreg(const int32_t* dem, int lines, int pixels, double lat0,
+                    double lon0, double dlat, double dlon, int nodata,
+                    const OrbitDev& m, const OrbitDev& s, int xmin, int xmax,
+                    int ymin, int ymax, double* d_partials, int nblocks,
+                    double* d_lp, hipStream_t st);
+int geocoreg_blocks(long posts);
+constexpr int kGeoSums = 34;   // 21 N + 6 rhs_az + 6 rhs_rg + count
+
+}  // namespace tops

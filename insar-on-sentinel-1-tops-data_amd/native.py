@@ -1,0 +1,143 @@
+"""ctypes bindings of libtops_mi355x.so (the C-ABI in include/tops_abi.h).
+
+The library is built in-tree by the package Makefile.  Loading fails loudly
+when it is missing: there is no CPU fallback for any entry point.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtops_mi355x.so")
+SYNTH_PATH = os.path.join(HERE, "libtops_synth.so")
+
+TOPS_OK = 0
+TOPS_ERR_ARG = 1
+TOPS_ERR_HIP = 2
+TOPS_ERR_NOT_PD = 3
+TOPS_ERR_NODEVICE = 4
+TOPS_ERR_ALLOC = 5
+MODE_FAST = 0
+MODE_FP64 = 1
+
+# every symbol declared in include/tops_abi.h
+EXPORTS = [
+    "tops_last_error", "tops_last_error_message", "tops_clear_error",
+    "GeoCoreg_warp_cuda", "DerampDemodResample", "DerampDemodResample_ESD",
+    "ResampleFirstBurst", "EstESDShifts", "CoherenceEst", "CoherenceEst_ESD",
+    "tops_ctx_create", "tops_ctx_destroy", "tops_resample_burst",
+    "tops_coherence", "tops_esd_shifts", "tops_geocoreg",
+]
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class BurstGeom(C.Structure):
+    _fields_ = [("master_box", C.c_int * 4), ("slave_box", C.c_int * 4),
+                ("first_line_in_burst", C.c_int), ("samples_per_burst", C.c_int),
+                ("az_time_interval", C.c_double), ("az_shift", C.c_double),
+                ("cpm_az", C.c_double * 6), ("cpm_rg", C.c_double * 6)]
+
+
+class OrbitGeom(C.Structure):
+    _fields_ = [("coef_x", _dp), ("coef_y", _dp), ("coef_z", _dp), ("ncoef", C.c_int),
+                ("mid_time", C.c_double), ("az_time_interval", C.c_double),
+                ("first_line_time", C.c_double), ("slr_time_first_pixel", C.c_double),
+                ("range_spacing", C.c_double), ("az_init", C.c_double),
+                ("line_offset", C.c_int)]
+
+
+_lib = None
+_synth = None
+
+
+def load():
+    """Load libtops_mi355x.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    L.tops_last_error.restype = C.c_int
+    L.tops_last_error_message.restype = C.c_char_p
+    L.tops_clear_error.restype = None
+    L.tops_ctx_create.restype = _vp
+    L.tops_ctx_create.argtypes = [_fp, _fp, C.c_int]
+    L.tops_ctx_destroy.argtypes = [_vp]
+    L.tops_ctx_destroy.restype = None
+    L.tops_resample_burst.restype = C.c_int
+    L.tops_resample_burst.argtypes = [_vp, C.POINTER(BurstGeom), _vp, _vp, _vp, _vp, _vp,
+                                      C.c_int, _vp]
+    L.tops_coherence.restype = C.c_int
+    L.tops_coherence.argtypes = [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_int, _vp, C.c_int, _vp]
+    L.tops_esd_shifts.restype = C.c_int
+    L.tops_esd_shifts.argtypes = [_vp, _vp, _vp, _vp, _ip, C.c_int, C.c_int, C.c_int,
+                                  C.c_int, C.c_float, C.c_double, C.c_double, _dp, _dp, _vp]
+    L.tops_geocoreg.restype = C.c_int
+    L.tops_geocoreg.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_int, C.POINTER(OrbitGeom),
+                                C.POINTER(OrbitGeom), C.c_int, C.c_int, C.c_int, C.c_int,
+                                _dp, _dp, C.POINTER(C.c_int64), _vp, _vp]
+    # reference entry points
+    L.DerampDemodResample.restype = None
+    L.DerampDemodResample.argtypes = [_vp, _dp, _dp, C.c_double, _vp, _fp, _fp, C.c_int,
+                                      C.c_int, C.c_int, _ip, _ip, C.c_int, C.c_int,
+                                      C.c_double, _dp, _dp, _dp, C.c_int]
+    L.DerampDemodResample_ESD.restype = _vp
+    L.DerampDemodResample_ESD.argtypes = L.DerampDemodResample.argtypes
+    L.ResampleFirstBurst.restype = _vp
+    L.ResampleFirstBurst.argtypes = [_vp, C.c_int, C.c_int]
+    L.CoherenceEst.restype = None
+    L.CoherenceEst.argtypes = [C.c_int] * 8 + [_vp, _vp, _fp]
+    L.CoherenceEst_ESD.restype = None
+    L.CoherenceEst_ESD.argtypes = L.CoherenceEst.argtypes
+    L.EstESDShifts.restype = None
+    L.EstESDShifts.argtypes = [_vp, _vp, _vp, _vp, _ip, _dp, C.c_int, C.c_int, C.c_int,
+                               C.c_int, C.c_int, C.c_int, C.c_float, C.c_double, C.c_double]
+    L.GeoCoreg_warp_cuda.restype = None
+    L.GeoCoreg_warp_cuda.argtypes = (
+        [C.c_int] * 6 + [C.c_double, C.c_double, C.c_int, C.c_int, _dp, _dp, C.c_double,
+                         C.c_double, C.c_int] + [C.c_double] * 7 + [C.c_int, C.c_int, _dp, _dp]
+        + [C.c_int] * 4 + [C.c_double] * 4 + [C.c_bool, C.c_bool, _ip] + [_dp] * 6
+        + [C.c_double, C.c_double])
+    _lib = L
+    return L
+
+
+def check(rc, what="tops"):
+    """Raise RuntimeError with the library's message on a non-zero status."""
+    if rc != TOPS_OK:
+        msg = load().tops_last_error_message().decode(errors="replace")
+        raise RuntimeError(f"{what}: status {rc}: {msg}")
+
+
+def last_error():
+    L = load()
+    return L.tops_last_error(), L.tops_last_error_message().decode(errors="replace")
+
+
+def load_synth():
+    global _synth
+    if _synth is not None:
+        return _synth
+    if not os.path.exists(SYNTH_PATH):
+        raise RuntimeError(f"{SYNTH_PATH} is missing: build it first")
+    S = C.CDLL(SYNTH_PATH)
+    S.tsyn_burst.restype = None
+    S.tsyn_burst.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_double,
+                             C.c_int64, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double,
+                             _dp, _dp, _dp, C.c_double, C.c_uint64, _vp]
+    S.tsyn_dem.restype = None
+    S.tsyn_dem.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int, C.c_int, C.c_double,
+                           C.c_double, C.c_int, C.c_int, _vp]
+    _synth = S
+    return S
+
+
+def ptr(a, ctype=C.c_double):
+    """ctypes pointer to a contiguous numpy array."""
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,182 @@
+"""GPU parity of deramp -> resample -> reramp and coherence against the FP64
+oracle (DerampAndResample.cu:348-520, EstCoh.cu:413-456 semantics).
+
+Tolerances: resampled samples max|dz|/RMS|z| <= 1e-5, coherence max |dc|
+<= 1e-4 absolute.  The error is printed for every case.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_RESAMPLE = 1e-5
+TOL_COH = 1e-4
+
+
+@pytest.fixture(scope="module")
+def c1(tops, orc):
+    pair = tops.synth.single_burst_small()
+    m = pair.master
+    M = pair.burst(0, "master")
+    S = pair.burst(0, "slave")
+    kt, tref, fdc, _ = pair.params()
+    kaz, krg = orc.lut(prf=m.prf, abw=m.azimuth_bandwidth, rsr2x=2 * m.range_sampling_rate)
+    ca, cr = pair.cpm(0)
+    return dict(pair=pair, M=M, S=S, kt=kt[0].copy(), tref=tref[0].copy(), fdc=fdc[0].copy(),
+                kaz=kaz, krg=krg, ca=ca, cr=cr, L=m.lines_per_burst, P=m.samples_per_burst,
+                dt=m.az_time_interval)
+
+
+class Resampler:
+    def __init__(self, tops, dev, d):
+        self.N = tops.native
+        self.L = self.N.load()
+        self.dev = dev
+        self.ctx = self.L.tops_ctx_create(self.N.ptr(d["kaz"], C.c_float),
+                                          self.N.ptr(d["krg"], C.c_float), 12)
+        assert self.ctx, self.N.last_error()
+
+    def close(self):
+        self.L.tops_ctx_destroy(self.ctx)
+
+    def run(self, slave, mbox, sbox, first_line, dt, az_shift, ca, cr, kt, tref, fdc, mode,
+            spb=None):
+        N, dev = self.N, self.dev
+        g = N.BurstGeom((C.c_int * 4)(*mbox), (C.c_int * 4)(*sbox), first_line,
+                        spb if spb is not None else len(kt), dt, az_shift,
+                        (C.c_double * 6)(*ca), (C.c_double * 6)(*cr))
+        ds = dev.put(slave)
+        dk, dtr, dfd = dev.put(kt), dev.put(tref), dev.put(fdc)
+        ml, mp = mbox[3] - mbox[2] + 1, mbox[1] - mbox[0] + 1
+        do = dev.empty(ml * mp * 8)
+        print(f"  resample mode={mode} mbox={mbox} sbox={sbox} shift={az_shift}", flush=True)
+        rc = self.L.tops_resample_burst(self.ctx, C.byref(g), ds.data_ptr(), dk.data_ptr(),
+                                        dtr.data_ptr(), dfd.data_ptr(), do.data_ptr(), mode, None)
+        N.check(rc, "tops_resample_burst")
+        out = dev.get(do, np.complex64, (ml, mp))
+        return out, do
+
+
+@pytest.fixture(scope="module")
+def rs(tops, dev, c1):
+    r = Resampler(tops, dev, c1)
+    yield r
+    r.close()
+
+
+def _oracle(orc, d, mbox, sbox, first_line, az_shift, ca, cr, S=None):
+    return orc.resample(d["S"] if S is None else S, mbox, sbox, first_line, d["dt"], az_shift, ca,
+                        cr, d["kaz"], d["krg"], d["kt"], d["tref"], d["fdc"])
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["fast", "fp64"])
+def test_resample_small_burst(orc, c1, rs, mode):
+    L, P = c1["L"], c1["P"]
+    box = [0, P - 1, 0, L - 1]
+    ref = _oracle(orc, c1, box, box, 0, 0.0, c1["ca"], c1["cr"])
+    got, _ = rs.run(c1["S"], box, box, 0, c1["dt"], 0.0, c1["ca"], c1["cr"], c1["kt"],
+                    c1["tref"], c1["fdc"], mode)
+    e = rel_err(got, ref)
+    p999 = np.quantile(np.abs(got - ref) / np.maximum(np.abs(ref), 1e-30), 0.999)
+    print(f"  resample {['fast', 'fp64'][mode]}: max|dz|/rms = {e:.3e}, p99.9 pointwise rel = {p999:.3e}")
+    assert e <= TOL_RESAMPLE
+
+
+def test_resample_shift_and_offset_boxes(orc, c1, rs):
+    """Burst index > 0 (absolute slave lines), an ESD azimuth shift and a
+    master box not starting at column 0."""
+    L, P = c1["L"], c1["P"]
+    b = 3
+    mbox = [16, P - 1, b * L, (b + 1) * L - 1]
+    sbox = [0, P - 1, b * L, (b + 1) * L - 1]
+    # Cpm for absolute slave lines of burst b, master columns offset by 16
+    ca = c1["ca"].copy()
+    ca[0] += b * L
+    shift = 0.37
+    ref = _oracle(orc, c1, mbox, sbox, b * L, shift, ca, c1["cr"])
+    got, _ = rs.run(c1["S"], mbox, sbox, b * L, c1["dt"], shift, ca, c1["cr"], c1["kt"],
+                    c1["tref"], c1["fdc"], 0)
+    e = rel_err(got, ref)
+    print(f"  shifted burst {b}: max|dz|/rms = {e:.3e}")
+    assert e <= TOL_RESAMPLE
+
+
+def test_resample_out_of_box_is_zero(orc, c1, rs):
+    """Offsets pushing part of the burst outside the slave box: zero region
+    and edge band must match the oracle exactly in structure."""
+    L, P = c1["L"], c1["P"]
+    box = [0, P - 1, 0, L - 1]
+    ca = c1["ca"].copy()
+    cr = c1["cr"].copy()
+    ca[0] += 40.3       # 40 lines down: the last lines fall off the box
+    cr[0] -= 25.6       # 25 pixels left: the first columns fall off
+    ref = _oracle(orc, c1, box, box, 0, 0.0, ca, cr)
+    got, _ = rs.run(c1["S"], box, box, 0, c1["dt"], 0.0, ca, cr, c1["kt"], c1["tref"],
+                    c1["fdc"], 0)
+    zero_ref = np.abs(ref) == 0
+    assert zero_ref.sum() > 0
+    assert np.all(got[zero_ref] == 0)
+    e = rel_err(got, ref)
+    print(f"  out-of-box case: {zero_ref.sum()} zeros, max|dz|/rms = {e:.3e}")
+    assert e <= TOL_RESAMPLE
+
+
+def test_coherence_small_burst(tops, orc, dev, c1, rs):
+    L, P = c1["L"], c1["P"]
+    box = [0, P - 1, 0, L - 1]
+    got, dres = rs.run(c1["S"], box, box, 0, c1["dt"], 0.0, c1["ca"], c1["cr"], c1["kt"],
+                       c1["tref"], c1["fdc"], 0)
+    N = tops.native
+    Lb = N.load()
+    dm = dev.put(c1["M"])
+    for (waz, wrg) in [(3, 10), (6, 24), (1, 1), (5, 7)]:
+        dx, dy = (wrg - 1) // 2, (waz - 1) // 2
+        ref = orc.coherence(c1["M"], got.astype(np.complex128), dx, dy, wrg, waz)
+        for mode in (0, 1):
+            dc = dev.empty(L * P * 4)
+            rc = Lb.tops_coherence(dm.data_ptr(), dres.data_ptr(), P, L, dx, dy, wrg, waz,
+                                   dc.data_ptr(), mode, None)
+            N.check(rc, "tops_coherence")
+            coh = dev.get(dc, np.float32, (L, P))
+            e = float(np.max(np.abs(coh - ref)))
+            print(f"  coherence {waz}x{wrg} mode={mode}: max|dc| = {e:.3e}, mean = {coh.mean():.4f}")
+            assert e <= TOL_COH
+
+
+def test_reference_entry_points(tops, orc, c1):
+    """DerampDemodResample / _ESD + CoherenceEst through host buffers, as
+    Func.cpp:2382 and 2701-2716 call them."""
+    N = tops.native
+    Lb = N.load()
+    L, P = c1["L"], c1["P"]
+    box = (C.c_int * 4)(0, P - 1, 0, L - 1)
+    out = np.zeros((L, P), np.complex64)
+    S = np.ascontiguousarray(c1["S"])
+    args = [S.ctypes.data, N.ptr(c1["ca"]), N.ptr(c1["cr"]), 0.0, out.ctypes.data,
+            N.ptr(c1["kaz"], C.c_float), N.ptr(c1["krg"], C.c_float), 0, P, L, box, box, L, P,
+            c1["dt"], N.ptr(c1["kt"]), N.ptr(c1["tref"]), N.ptr(c1["fdc"]), 12]
+    Lb.tops_clear_error()
+    Lb.DerampDemodResample(*args)
+    assert Lb.tops_last_error() == 0, N.last_error()
+    ref = _oracle(orc, c1, [0, P - 1, 0, L - 1], [0, P - 1, 0, L - 1], 0, 0.0, c1["ca"], c1["cr"])
+    assert rel_err(out, ref) <= TOL_RESAMPLE
+    out2 = np.zeros((L, P), np.complex64)
+    args[4] = out2.ctypes.data
+    dptr = Lb.DerampDemodResample_ESD(*args)
+    assert dptr, N.last_error()
+    assert np.array_equal(out, out2)
+    coh = np.zeros((L, P), np.float32)
+    M = np.ascontiguousarray(c1["M"])
+    Lb.CoherenceEst(0, 0, P, L, 4, 1, 10, 3, M.ctypes.data, dptr, N.ptr(coh, C.c_float))
+    assert Lb.tops_last_error() == 0, N.last_error()
+    cref = orc.coherence(M, out.astype(np.complex128), 4, 1, 10, 3)
+    assert float(np.max(np.abs(coh - cref))) <= TOL_COH
+    # npoints other than 12 is refused with a status, not a crash
+    args[-1] = 6
+    Lb.DerampDemodResample(*args)
+    assert Lb.tops_last_error() == N.TOPS_ERR_ARG
+    Lb.tops_clear_error()
