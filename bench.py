@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of the TOPS burst hot path on one MI355X.
+
+A step is one pass of deramp -> 12x12 sinc resample -> reramp -> coherence
+over one synthetic burst already resident in HBM (default workload: a full
+IW1 burst, 1500 az x 21000 rg CInt16, 3 az x 10 rg coherence window).  The
+metric counts master-grid output samples per second.
+
+Printed JSON line (one): metric, value (Msamples/s), roofline of the
+dominant kernel (the fused resample kernel; HIP-event durations on the
+stream it runs on), the plain all-FP64 GPU form measured in the same run
+(vs_plain_fp64), the error of the measured output against the FP64 oracle,
+and the CPU baseline (the FP64 oracle on the host cores).
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "burst Msamples/s (deramp+resample+coh) vs plain FP64 on MI355X; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per output sample: the resample kernel reads the CInt16
+# slave once (4 B) and writes the CF32 result (8 B); the whole path adds the
+# CInt16 master read (4 B) and the float coherence write (4 B): 20 B.
+RESAMPLE_BYTES = 12
+PATH_BYTES = 20
+
+
+def load_package():
+    import __graft_entry__ as ge
+    return ge._import_package()
+
+
+def workloads(tops, name):
+    s = tops.synth
+    if name == "single_burst_iw1":
+        return s.single_burst_iw1()
+    if name == "single_burst_small":
+        return s.single_burst_small()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def read_traffic(kernel, workload):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), from the committed
+    profile summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="single_burst_iw1")
+    ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    tops = load_package()
+    N = tops.native
+    L = N.load()
+    pair = workloads(tops, args.workload)
+    m = pair.master
+    lines, pixels = m.lines_per_burst, m.samples_per_burst
+    samples = lines * pixels
+    print(f"[bench] workload {args.workload}: {lines} x {pixels}", file=sys.stderr, flush=True)
+
+    # host stage layer: LUT, orbit fit, deramp parameters (InKernelInit,
+    # Polyfit_Orbit, ComputeDerampPara) through the library
+    kaz, krg = N.lut_init(12, m.prf, m.azimuth_bandwidth, 2 * m.range_sampling_rate)
+    t, x, y, z = tops.synth.orbit_polyfit_inputs(pair.sorbit)
+    cx, cy, cz = N.polyfit_orbit(t, x, y, z)
+    speed = L.tops_orbit_speed(0.5 * (pair.slave.first_line_time + pair.slave.last_line_time),
+                               t[len(t) // 2], N.ptr(cx), N.ptr(cy), N.ptr(cz), len(cx))
+    kt, tref, fdc, _ = N.deramp_params(pair.slave, speed)
+    ca, cr = pair.cpm(0)
+    M = pair.burst(0, "master")
+    S = pair.burst(0, "slave")
+    print("[bench] inputs generated", file=sys.stderr, flush=True)
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sp = C.c_void_p(stream.cuda_stream)
+    d = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+         for k, v in dict(S=S, M=M, kt=kt[0], tref=tref[0], fdc=fdc[0]).items()}
+    out = torch.empty(samples * 2, dtype=torch.float32, device=dev)
+    coh = torch.empty(samples, dtype=torch.float32, device=dev)
+    ctx = L.tops_ctx_create(N.ptr(kaz, C.c_float), N.ptr(krg, C.c_float), 12)
+    if not ctx:
+        raise RuntimeError(N.last_error())
+    box = [0, pixels - 1, 0, lines - 1]
+    g = N.BurstGeom((C.c_int * 4)(*box), (C.c_int * 4)(*box), 0, pixels, m.az_time_interval, 0.0,
+                    (C.c_double * 6)(*ca), (C.c_double * 6)(*cr))
+    win_az, win_rg = pair.coh_win
+    dx, dy = (win_rg - 1) // 2, (win_az - 1) // 2
+
+    def step(mode, ev=None):
+        if ev:
+            ev[0].record(stream)
+        N.check(L.tops_resample_burst(ctx, C.byref(g), d["S"].data_ptr(), d["kt"].data_ptr(),
+                                      d["tref"].data_ptr(), d["fdc"].data_ptr(), out.data_ptr(),
+                                      mode, sp), "resample")
+        if ev:
+            ev[1].record(stream)
+        N.check(L.tops_coherence(d["M"].data_ptr(), out.data_ptr(), pixels, lines, dx, dy,
+                                 win_rg, win_az, coh.data_ptr(), mode, sp), "coherence")
+        if ev:
+            ev[2].record(stream)
+
+    def timed(mode, warmup, steps):
+        for _ in range(warmup):
+            step(mode)
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            step(mode, evs[k])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rs = [e[0].elapsed_time(e[1]) for e in evs]
+        ch = [e[1].elapsed_time(e[2]) for e in evs]
+        return (t1 - t0) / steps, float(np.mean(rs)) * 1e-3, float(np.mean(ch)) * 1e-3
+
+    # plain all-FP64 GPU form, same inputs, same stream
+    fp64_step, fp64_rs, fp64_coh = timed(N.MODE_FP64, 1, max(1, args.fp64_steps))
+    # the measured fast path
+    step_s, rs_s, coh_s = timed(N.MODE_FAST, args.warmup, args.steps)
+    value = samples / step_s / 1e6
+    got = out.cpu().numpy().view(np.complex64).reshape(lines, pixels)
+    got_coh = coh.cpu().numpy().reshape(lines, pixels)
+    print(f"[bench] fast {step_s * 1e3:.3f} ms/step (resample {rs_s * 1e3:.3f} ms, coherence "
+          f"{coh_s * 1e3:.3f} ms); fp64 {fp64_step * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
+
+    achieved = RESAMPLE_BYTES * samples / rs_s / 1e9
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "burst_lines": lines, "burst_samples": pixels,
+                   "taps": 12, "coh_window": [win_az, win_rg],
+                   "precision": "FP32 sinc MACs and window sums, FP64 phase reduced mod 2pi",
+                   "parallelism": "single GPU"},
+        "vs_plain_fp64": round(fp64_step / step_s, 3),
+        "plain_fp64_msamples_per_s": round(samples / fp64_step / 1e6, 2),
+        "kernels_ms": {"resample_fast_kernel": round(rs_s * 1e3, 4),
+                       "coherence_fast_kernel": round(coh_s * 1e3, 4),
+                       "plain_fp64_resample": round(fp64_rs * 1e3, 4),
+                       "plain_fp64_coherence": round(fp64_coh * 1e3, 4)},
+        "path_hbm_frac": round(PATH_BYTES * samples / step_s / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"kernel": "resample_fast_kernel", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic("resample_fast_kernel", args.workload),
+                     "algorithmic_bytes_per_launch": RESAMPLE_BYTES * samples},
+    }
+
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        box_l = [0, pixels - 1, 0, lines - 1]
+        orc.resample(S, box_l, box_l, 0, m.az_time_interval, 0.0, ca, cr, kaz, krg, kt[0],
+                     tref[0], fdc[0], 0, 64)   # warm-up (page faults, thread pool)
+        t0 = time.perf_counter()
+        ref = orc.resample(S, box_l, box_l, 0, m.az_time_interval, 0.0, ca, cr, kaz, krg, kt[0],
+                           tref[0], fdc[0])
+        cref = orc.coherence(M, ref, dx, dy, win_rg, win_az)
+        t_cpu = time.perf_counter() - t0
+        rms = np.sqrt(np.mean(np.abs(ref) ** 2))
+        err = float(np.max(np.abs(got - ref)) / rms)
+        cerr = float(np.max(np.abs(got_coh - cref)))
+        result["cpu_baseline"] = {"value": round(samples / t_cpu / 1e6, 3), "unit": "Msamples/s",
+                                  "cores": threads, "kind": "port",
+                                  "sample": f"the full {lines}x{pixels} burst, deramp+resample+"
+                                            f"reramp+coherence in FP64 (oracle/tops_oracle.c, "
+                                            f"OpenMP), {t_cpu:.2f} s",
+                                  "cpu": cpu_model()}
+        result["error_vs_oracle"] = {"resample_max_abs_over_rms": err, "tol_resample": 1e-5,
+                                     "coherence_max_abs": cerr, "tol_coherence": 1e-4}
+    L.tops_ctx_destroy(ctx)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

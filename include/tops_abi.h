@@ -207,6 +207,43 @@ int tops_geocoreg(const int32_t* d_dem, int lines, int pixels, double lat0,
                   int xmin, int xmax, int ymin, int ymax, double* cpm_az,
                   double* cpm_rg, int64_t* n_used, double* d_lp, void* stream);
 
+/* ======================================================================== */
+/* Part 3: host stage layer                                                 */
+/* ======================================================================== */
+
+/* Replaces InKernelInit, Func.cpp:2105-2158: fills kernel_az/kernel_rg
+ * [2048 x npoints] (chi_az = prf/abw, chi_rg = (rsr2x/2)/abw as the
+ * reference, Func.cpp:2110-2111). */
+int tops_lut_init(int npoints, double prf, double abw, double rsr2x, float* kernel_az,
+                  float* kernel_rg);
+
+/* Replaces Polyfit_Orbit, Func.cpp:1050-1126: returns the number of
+ * coefficients (min(5, n-2) + 1), 0 on failure. */
+int tops_polyfit_orbit(int n, const double* t, const double* x, const double* y,
+                       const double* z, double* coef_x, double* coef_y, double* coef_z);
+
+/* |velocity| at absolute time t (getVelocity, Func.cpp:1279-1300). */
+double tops_orbit_speed(double t, double mid_time, const double* cx, const double* cy,
+                        const double* cz, int ncoef);
+
+/* Replaces the formulas of ComputeDerampPara, Func.cpp:1868-2099 (the
+ * annotation XML reading is not part of this library): dc/fm are
+ * [n_bursts][4] = (t0, c0, c1, c2); outputs [n_bursts x spb]. */
+int tops_deramp_params(int n_bursts, int spb, double slr_time_first_pixel,
+                       double range_spacing, int lines_per_burst, double az_time_interval,
+                       int first_valid_pixel, const double* dc, const double* fm,
+                       double steering_rate_deg, double wavelength, double orbit_speed,
+                       double* kt, double* tref, double* fdc, double* ka);
+
+/* Replaces EstOverlapSize, Func.cpp:2935-2960; returns the overlap count. */
+int tops_overlap_sizes(const double* first_line_time, const double* last_line_time,
+                       int burst0, int burstN, double az_time_interval, int* sizes,
+                       int* total, int* maximum);
+
+/* Replaces GetSpectralSep, Func.cpp:2962-2976. */
+double tops_spectral_sep(const double* kt, int n_bursts, int spb, int lines_per_burst,
+                         double az_time_interval);
+
 #ifdef __cplusplus
 }
 #endif

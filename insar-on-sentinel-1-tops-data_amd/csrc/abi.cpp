@@ -318,6 +318,8 @@ int tops_geocoreg(const int32_t* d_dem, int lines, int pixels, double lat0, doub
     return TOPS_OK;
 }
 
+}  // extern "C"
+
 // ---------------------------------------------------------------------------
 // reference entry points (host buffers)
 // ---------------------------------------------------------------------------
@@ -441,6 +443,8 @@ int ref_coherence(int Dw, int Dh, int dx, int dy, int cohWinRg, int cohWinAz,
 }
 
 }  // namespace
+
+extern "C" {
 
 void DerampDemodResample(tops_ci16* SlaveArray, double* CpmAz, double* CpmRg, double AzimuthShift,
                          tops_cf32* output, float* KernelAz, float* KernelRg, int sBurstIdx,

@@ -26,6 +26,8 @@ EXPORTS = [
     "ResampleFirstBurst", "EstESDShifts", "CoherenceEst", "CoherenceEst_ESD",
     "tops_ctx_create", "tops_ctx_destroy", "tops_resample_burst",
     "tops_coherence", "tops_esd_shifts", "tops_geocoreg",
+    "tops_lut_init", "tops_polyfit_orbit", "tops_orbit_speed", "tops_deramp_params",
+    "tops_overlap_sizes", "tops_spectral_sep",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -104,8 +106,60 @@ def load():
                          C.c_double, C.c_int] + [C.c_double] * 7 + [C.c_int, C.c_int, _dp, _dp]
         + [C.c_int] * 4 + [C.c_double] * 4 + [C.c_bool, C.c_bool, _ip] + [_dp] * 6
         + [C.c_double, C.c_double])
+    # host stage layer
+    L.tops_lut_init.restype = C.c_int
+    L.tops_lut_init.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _fp, _fp]
+    L.tops_polyfit_orbit.restype = C.c_int
+    L.tops_polyfit_orbit.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.tops_orbit_speed.restype = C.c_double
+    L.tops_orbit_speed.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp, C.c_int]
+    L.tops_deramp_params.restype = C.c_int
+    L.tops_deramp_params.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                     C.c_double, C.c_int, _dp, _dp, C.c_double, C.c_double,
+                                     C.c_double, _dp, _dp, _dp, _dp]
+    L.tops_overlap_sizes.restype = C.c_int
+    L.tops_overlap_sizes.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_double, _ip, _ip, _ip]
+    L.tops_spectral_sep.restype = C.c_double
+    L.tops_spectral_sep.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double]
     _lib = L
     return L
+
+
+def lut_init(npoints, prf, abw, rsr2x):
+    """InKernelInit (Func.cpp:2105-2158) through the library."""
+    import numpy as np
+    kaz = np.zeros((2048, npoints), np.float32)
+    krg = np.zeros((2048, npoints), np.float32)
+    check(load().tops_lut_init(npoints, prf, abw, rsr2x, ptr(kaz, C.c_float), ptr(krg, C.c_float)),
+          "tops_lut_init")
+    return kaz, krg
+
+
+def polyfit_orbit(t, x, y, z):
+    """Polyfit_Orbit (Func.cpp:1050-1126) through the library."""
+    import numpy as np
+    cx, cy, cz = np.zeros(6), np.zeros(6), np.zeros(6)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (t, x, y, z)]
+    n = load().tops_polyfit_orbit(len(t), *[ptr(a) for a in arrs], ptr(cx), ptr(cy), ptr(cz))
+    if n == 0:
+        check(load().tops_last_error(), "tops_polyfit_orbit")
+    return cx[:n].copy(), cy[:n].copy(), cz[:n].copy()
+
+
+def deramp_params(sw, speed):
+    """ComputeDerampPara formulas through the library: kt, tref, fdc, ka
+    [n_bursts x samples_per_burst] for a synth.Swath."""
+    import numpy as np
+    nb, spb = sw.n_bursts, sw.samples_per_burst
+    out = [np.zeros((nb, spb)) for _ in range(4)]
+    dc = np.ascontiguousarray(sw.dc, dtype=np.float64)
+    fm = np.ascontiguousarray(sw.fm, dtype=np.float64)
+    check(load().tops_deramp_params(nb, spb, sw.slr_time_first_pixel, sw.range_spacing,
+                                    sw.lines_per_burst, sw.az_time_interval,
+                                    sw.first_valid_pixel, ptr(dc), ptr(fm), sw.steering_rate_deg,
+                                    sw.wavelength, speed, *[ptr(a) for a in out]),
+          "tops_deramp_params")
+    return tuple(out)
 
 
 def check(rc, what="tops"):

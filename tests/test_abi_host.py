@@ -1,0 +1,98 @@
+"""C-ABI checks that need no GPU: libtops_mi355x.so loads, exports every
+function include/tops_abi.h declares, and its host stage-layer functions
+(InKernelInit, Polyfit_Orbit, ComputeDerampPara formulas, EstOverlapSize,
+GetSpectralSep) agree with the oracle."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "tops_abi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    skip = {"if", "sizeof", "defined"}
+    return sorted({n for n in names if n not in skip})
+
+
+def test_library_exports_header(tops):
+    L = tops.native.load()
+    names = _header_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(tops.native.EXPORTS) <= set(names)
+
+
+def test_status_api(tops):
+    L = tops.native.load()
+    L.tops_clear_error()
+    assert L.tops_last_error() == 0
+    # argument errors are reported without touching a device
+    rc = L.tops_lut_init(0, 1.0, 1.0, 1.0, None, None)
+    assert rc == tops.native.TOPS_ERR_ARG
+    assert L.tops_last_error() == tops.native.TOPS_ERR_ARG
+    assert b"tops_lut_init" in L.tops_last_error_message()
+    L.tops_clear_error()
+
+
+def test_lut_matches_oracle(tops, orc):
+    for args in ((12, 1 / 2.055556e-3, 327.0, 2 * 64.34523076923077e6),
+                 (12, 1 / 2.758e-3, 313.0, 2 * 64.34523076923077e6)):
+        kaz, krg = tops.native.lut_init(*args)
+        oaz, org = orc.lut(*args)
+        # same formula in FP64, stored as float: equal to the last float bit
+        assert np.max(np.abs(kaz.view(np.int32) - oaz.view(np.int32))) <= 1
+        assert np.max(np.abs(krg.view(np.int32) - org.view(np.int32))) <= 1
+
+
+def test_polyfit_matches_oracle(tops, orc):
+    orbit = tops.synth.make_orbit(30.0, 100.0, 43200.0)
+    t, x, y, z = tops.synth.orbit_polyfit_inputs(orbit)
+    a = tops.native.polyfit_orbit(t, x, y, z)
+    b = orc.polyfit_orbit(t, x, y, z)
+    for ca, cb in zip(a, b):
+        assert len(ca) == len(cb) == 6
+        # evaluate both fits over the orbit span: same track to 1e-6 m
+        for tn in np.linspace(-4.5, 4.5, 11):
+            pa = sum(ca[j] * tn ** j for j in range(6))
+            pb = sum(cb[j] * tn ** j for j in range(6))
+            assert abs(pa - pb) < 1e-6
+
+
+def test_deramp_params_match_oracle(tops, orc):
+    pair = tops.synth.subswath_iw1(n_bursts=3, lines=200, pixels=300)
+    m = pair.master
+    m.dc[1, 3] = 1e9      # a quadratic Doppler term in burst 1
+    m.fm[2, 2] = -3e5     # a linear FM-rate term in burst 2
+    got = tops.native.deramp_params(m, 7421.3)
+    ref = orc.deramp_params(m.n_bursts, m.samples_per_burst, m.slr_time_first_pixel,
+                            m.range_spacing, m.lines_per_burst, m.az_time_interval,
+                            m.first_valid_pixel, m.dc, m.fm, m.steering_rate_deg, m.wavelength,
+                            7421.3)
+    for g, r in zip(got, ref):
+        assert np.allclose(g, r, rtol=1e-13, atol=1e-13)
+
+
+def test_overlap_and_spectral_sep_match_oracle(tops, orc):
+    pair = tops.synth.subswath_iw1(n_bursts=4, lines=300, pixels=64)
+    m = pair.master
+    L = tops.native.load()
+    sizes = np.zeros(8, np.int32)
+    tot, mx = C.c_int(0), C.c_int(0)
+    n = L.tops_overlap_sizes(tops.native.ptr(m.burst_first_line_time),
+                             tops.native.ptr(m.burst_last_line_time), 0, 3, m.az_time_interval,
+                             sizes.ctypes.data_as(C.POINTER(C.c_int)), C.byref(tot), C.byref(mx))
+    osz, otot, omx = orc.overlap_sizes(m.burst_first_line_time, m.burst_last_line_time, 0, 3,
+                                       m.az_time_interval)
+    assert n == 3 and list(sizes[:3]) == list(osz) and tot.value == otot and mx.value == omx
+    assert list(osz) == [99, 99, 99]
+    kt = np.ascontiguousarray(pair.params()[0])
+    a = L.tops_spectral_sep(tops.native.ptr(kt), kt.shape[0], kt.shape[1], m.lines_per_burst,
+                            m.az_time_interval)
+    b = orc.spectral_sep(kt, m.lines_per_burst, m.az_time_interval)
+    assert abs(a - b) <= 1e-9 * abs(b)
