@@ -184,6 +184,36 @@ int tops_esd_shifts(const tops_ci16* d_m_for, const tops_ci16* d_m_back,
                     double spectral_sep, double az_time_interval,
                     double* shifts, double* sums, void* stream);
 
+/* ---- batched bursts (no counterpart in the reference, which runs one
+ * burst per call with a device reset in between) ---------------------- */
+typedef struct {
+    tops_burst_geom geom;
+    const tops_ci16* slave;      /* device [sLines x sPixels]                */
+    const double* kt;            /* device [samples_per_burst] (+ tref, fdc) */
+    const double* tref;
+    const double* fdc;
+    tops_cf32* out;              /* device [mLines x mPixels]                */
+    const tops_ci16* master;     /* device [mLines x mPixels]; NULL: no coherence */
+    float* coh;                  /* device [mLines x mPixels]                */
+} tops_burst_item;
+
+typedef struct tops_plan tops_plan;
+/* One plan = one resample launch over every burst's strips plus one
+ * coherence launch over every burst's tiles (window coh_win_az x
+ * coh_win_rg starting (win-1)/2 before the sample).  strip_rows <= 0 picks
+ * the default strip height. */
+tops_plan* tops_plan_create(tops_ctx* ctx, const tops_burst_item* items, int n_items,
+                            int coh_win_az, int coh_win_rg, int strip_rows);
+int tops_plan_run(tops_plan* plan, void* stream);
+/* Capture the plan's launches in a hipGraph on a non-default stream; later
+ * tops_plan_run calls replay it. */
+int tops_plan_capture_graph(tops_plan* plan, void* stream);
+/* Time the candidate strip heights on this plan and keep the fastest
+ * (cached in the file named by TOPS_TUNE_CACHE when set). */
+int tops_plan_tune(tops_plan* plan, void* stream, int* best_strip_rows);
+int tops_plan_strip_rows(const tops_plan* plan);
+void tops_plan_destroy(tops_plan* plan);
+
 /* Geocoreg orbit description for one acquisition (device-side copy is made). */
 typedef struct {
     const double* coef_x; const double* coef_y; const double* coef_z;
@@ -243,6 +273,76 @@ int tops_overlap_sizes(const double* first_line_time, const double* last_line_ti
 /* Replaces GetSpectralSep, Func.cpp:2962-2976. */
 double tops_spectral_sep(const double* kt, int n_bursts, int spb, int lines_per_burst,
                          double az_time_interval);
+
+/* SubSwathInfo subset (DataTypes.h:51-140) the stages read.  Arrays are
+ * host memory; the per-range arrays are [n_bursts x samples_per_burst] as
+ * ComputeDerampPara leaves them; slc is the burst-stacked raster
+ * [n_bursts * lines_per_burst x samples_per_burst] of the SLC. */
+typedef struct {
+    int lines_per_burst, samples_per_burst, n_bursts;
+    double az_time_interval, range_spacing, slr_time_first_pixel, wavelength;
+    const double* burst_first_line_time;   /* [n_bursts] */
+    const double* burst_last_line_time;    /* [n_bursts] */
+    const double* burst_ll;                /* [n_bursts][4] lat_min lat_max lon_min lon_max */
+    const double* doppler_rate;            /* kt */
+    const double* reference_time;          /* t_ref */
+    const double* doppler_centroid;        /* f_dc */
+    const tops_ci16* slc;
+} tops_swath;
+
+/* S1PreciseOrbit after Polyfit_Orbit (DataTypes.h:162-181). */
+typedef struct {
+    int num_points;
+    const double* az_time;                 /* [num_points] */
+    int ncoef;
+    const double* coef_x;
+    const double* coef_y;
+    const double* coef_z;
+} tops_orbit;
+
+/* In-memory DEM (what RefDem reads from a GeoTIFF, DataTypes.cpp:201-307):
+ * post (r, c) at (lat_max - r dlat, lon_min + c dlon) degrees. */
+typedef struct {
+    const int32_t* data;
+    int lines, pixels;
+    double lat_max, lon_min, dlat, dlon;
+    int nodata;
+} tops_dem;
+
+/* Replaces GeometricCoreg, Func.cpp:2215-2297.  cpm_az/cpm_rg receive
+ * [(mburstN - mburst0 + 1) x 6] coefficients. */
+int tops_geometric_coreg(const tops_swath* m, const tops_swath* s, const tops_orbit* mo,
+                         const tops_orbit* so, const tops_dem* dem, int mburst0, int mburstN,
+                         int sburst0, int sburstN, double* cpm_az, double* cpm_rg);
+
+/* Replaces DerampAndResample, Func.cpp:2326-2406: out (host) and/or d_out
+ * (device) receive the burst-stacked resampled slave
+ * [(mburstN - mburst0 + 1) * lpb x spb]. */
+int tops_deramp_and_resample(const tops_swath* m, const tops_swath* s, const double* cpm_az,
+                             const double* cpm_rg, int mburst0, int mburstN, int sburst0,
+                             int sburstN, const float* kernel_az, const float* kernel_rg,
+                             tops_cf32* out, tops_cf32* d_out);
+
+/* Replaces ESDAndCoh, Func.cpp:2496-2763 (reslave = the pass-1 result). */
+int tops_esd_and_coh(const tops_swath* m, const tops_swath* s, const double* cpm_az,
+                     const double* cpm_rg, int mburst0, int mburstN, int sburst0, int sburstN,
+                     const float* kernel_az, const float* kernel_rg, const tops_cf32* reslave,
+                     int esd_win_az, int esd_win_rg, float threshold, int coh_win_az,
+                     int coh_win_rg, tops_cf32* reslave_esd, float* coh, double* shifts);
+
+/* config.txt (Func.cpp:7-130 ReadInput, ConfigSet DataTypes.h:24-40):
+ * "key=value" lines; unknown keys are ignored. */
+typedef struct {
+    char process_dir[512];
+    char masterpath[512];
+    char slavepath[512];
+    char polarisation[32];
+    char precise_orbit_master[512];
+    char precise_orbit_slave[512];
+    char specific_dem_path[512];
+    int firstsubswath, lastsubswath, multi_az, multi_rg, burst0, burstN;
+} tops_config;
+int tops_read_config(const char* path, tops_config* cfg);
 
 #ifdef __cplusplus
 }

@@ -48,12 +48,11 @@ __global__ void __launch_bounds__(256) coherence_fp64_kernel(const tops_ci16* M,
     coh[q0] = pp > 0.0 ? (float)(sqrt(a0 * a0 + a1 * a1) / sqrt(pp)) : 0.0f;
 }
 
-__global__ void __launch_bounds__(256) coherence_fast_kernel(const tops_ci16* M, const tops_cf32* S, int w,
-                                      int h, int dx, int dy, int win_rg, int win_az,
-                                      float* coh)
+__device__ __forceinline__ void coherence_tile(const tops_ci16* M, const tops_cf32* S, int w,
+                                               int h, int dx, int dy, int win_rg, int win_az,
+                                               float* coh, int tx, int ty, float4* smem)
 {
-    extern __shared__ float4 smem[];
-    const int r0 = blockIdx.y * TW_ROWS, c0 = blockIdx.x * TW_COLS;
+    const int r0 = ty * TW_ROWS, c0 = tx * TW_COLS;
     WindowSums ws[4];
     tile_window_sums(M, S, w, h, r0, c0, dx, dy, win_rg, win_az, smem, ws, nullptr);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -69,6 +68,45 @@ __global__ void __launch_bounds__(256) coherence_fast_kernel(const tops_ci16* M,
         if (sqrtf(s.x * s.x + s.y * s.y) >= 0.001f) out = ws[k].coherence();
         coh[q] = out;
     }
+}
+
+__global__ void __launch_bounds__(256) coherence_fast_kernel(const tops_ci16* M, const tops_cf32* S, int w,
+                                      int h, int dx, int dy, int win_rg, int win_az,
+                                      float* coh)
+{
+    extern __shared__ float4 smem[];
+    coherence_tile(M, S, w, h, dx, dy, win_rg, win_az, coh, blockIdx.x, blockIdx.y, smem);
+}
+
+// Batched form: work[i] = {item, tile x, tile y, 0}.
+__global__ void __launch_bounds__(256) coherence_batch_kernel(const CohItem* items, const int4* work,
+                                                              int dx, int dy, int win_rg, int win_az)
+{
+    extern __shared__ float4 smem[];
+    const int4 wk = work[blockIdx.x];
+    const CohItem& it = items[wk.x];
+    coherence_tile(it.m, it.s, it.w, it.h, dx, dy, win_rg, win_az, it.coh, wk.y, wk.z, smem);
+}
+
+int coherence_tile_count(int w, int h, int* nx, int* ny)
+{
+    *nx = (w + TW_COLS - 1) / TW_COLS;
+    *ny = (h + TW_ROWS - 1) / TW_ROWS;
+    return *nx * *ny;
+}
+
+int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_work, int dx, int dy,
+                           int win_rg, int win_az, hipStream_t st)
+{
+    if (n_work <= 0) return TOPS_OK;
+    if (!tile_window_fits(win_rg, win_az))
+        return set_error(TOPS_ERR_ARG, "batched coherence: window %dx%d exceeds the LDS tile",
+                         win_az, win_rg);
+    hipLaunchKernelGGL(coherence_batch_kernel, dim3(n_work), dim3(256),
+                       tile_window_lds_bytes(win_rg, win_az), st, d_items, d_work, dx, dy, win_rg,
+                       win_az);
+    TOPS_HIP(hipGetLastError());
+    return TOPS_OK;
 }
 
 int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h, int dx,

@@ -166,22 +166,32 @@ int launch_resample_fp64(tops_ctx* ctx, const ResampleParams& p, hipStream_t s)
     return TOPS_OK;
 }
 
+// Fused fast form: column-walking waves
 // ======================================================================
-// Fused fast form
-// ======================================================================
-constexpr int FT_ROWS = 16;          // output rows per tile (4 per wave)
-constexpr int FT_COLS = 64;          // output cols per tile (one per lane)
-constexpr int FT_MAXY = 44;          // LDS slave window capacity (rows)
-constexpr int FT_MAXX = 100;         // LDS slave window capacity (cols)
+// One 64-lane workgroup owns a strip of WK_ROWS output rows x WK_OUT output
+// columns.  Lane l is bound to slave column ucol0 + l (ucol0 = the first
+// output's truncated slave column - 6) for the whole strip:
+//   * it keeps the deramped slave samples of rows wbase..wbase+15 of its
+//     column in registers; new rows are deramped on arrival with an FP64
+//     phasor recurrence along the column (exp(j phi) is advanced by
+//     exp(j (b + c (2y+1))), which is advanced by exp(j 2c)), anchored with
+//     an FP64 sincos at the strip start and after a resync;
+//   * per output row and per distinct azimuth key (Iy, fy) among the row's
+//     interior outputs it forms the 12-tap azimuth sum U of its column and
+//     the azimuth-interpolated phase PU = M0 a + (y0 M0 + M1) b +
+//     (y0^2 M0 + 2 y0 M1 + M2) c (FP64), and writes both to LDS;
+//   * each output then needs only its own 12 range taps over U and PU.
+// Outputs whose key or window the shared pass cannot serve (more than
+// WK_KEYS keys in one row segment, or a window outside the strip's 64
+// columns) are computed by interior_direct.
+constexpr int WK_OUT = 50;     // output columns per strip (64-lane wave)
+constexpr int WK_ROWS = 128;   // output rows per strip
+constexpr int WK_WIN = 16;     // register window rows
+constexpr int WK_KEYS = 4;     // azimuth keys per row served by the shared pass
 
 __device__ inline int wave_min(int v)
 {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ inline int wave_max(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
     return v;
 }
 
@@ -204,9 +214,10 @@ __device__ inline tops_cf32 reramp(float sr, float si, double phi)
     return o;
 }
 
-// Interior output computed straight from HBM (used when the tile's slave
-// window does not fit the LDS capacity: strongly warped offset fields).
-__device__ __noinline__ tops_cf32 interior_direct(const ResampleParams& p, int iy, int fy, int ix, int fx)
+// Interior output computed straight from HBM (outputs the shared pass does
+// not serve).
+__device__ __noinline__ tops_cf32 interior_direct(const ResampleParams& p, int iy, int fy, int ix,
+                                                 int fx)
 {
     float kl[kTaps], kp[kTaps];
     load_taps(p.kaz, fy, kl);
@@ -243,134 +254,299 @@ __device__ __noinline__ tops_cf32 interior_direct(const ResampleParams& p, int i
     return reramp(sr, si, phi);
 }
 
-__global__ void __launch_bounds__(256) resample_fast_kernel(ResampleParams p)
+// Column state of one lane: deramp recurrence along its slave column.
+struct ColumnWalker {
+    int xcol;            // absolute slave column
+    bool in_box;         // column inside the slave box
+    double a, b, c;      // phi(y) = a + b y + c y^2, y burst-relative line
+    double pr, pi;       // exp(j phi(ynext))
+    double ir, ii;       // exp(j (b + c (2 ynext + 1)))
+    double qr, qi;       // exp(j 2c)
+    int ynext;           // absolute slave line produced next
+
+    __device__ void anchor(const ResampleParams& p, int y_abs)
+    {
+        ynext = y_abs;
+        double y = (double)(y_abs - p.first_line);
+        double s, cc;
+        double ph = fma(y, fma(y, c, b), a);
+        sincos(fma(-rint(ph * kInvTwoPi), kTwoPi, ph), &s, &cc);
+        pr = cc; pi = s;
+        double d = fma(c, 2.0 * y + 1.0, b);
+        sincos(fma(-rint(d * kInvTwoPi), kTwoPi, d), &s, &cc);
+        ir = cc; ii = s;
+    }
+
+    // deramped sample of row ynext, then advance the recurrence
+    __device__ float2 next(const ResampleParams& p)
+    {
+        float2 d = make_float2(0.f, 0.f);
+        int y = ynext;
+        if (in_box && y >= p.sb[2] && y <= p.sb[3]) {
+            tops_ci16 s = p.slave[(size_t)(y - p.sb[2]) * p.s_pixels + (xcol - p.sb[0])];
+            float cr = (float)pr, ci = (float)pi;
+            float re = (float)s.re, im = (float)s.im;
+            d = make_float2(re * cr - im * ci, re * ci + im * cr);
+        }
+        double nr = pr * ir - pi * ii, ni = pr * ii + pi * ir;
+        pr = nr; pi = ni;
+        double jr = ir * qr - ii * qi, ji = ir * qi + ii * qr;
+        ir = jr; ii = ji;
+        ynext = y + 1;
+        return d;
+    }
+};
+
+template <int O>
+__device__ inline float2 az_dot(const float2 (&w)[WK_WIN], const float (&kl)[kTaps])
 {
-    __shared__ float2 sD[FT_MAXY * FT_MAXX];
-    __shared__ double sA[FT_MAXX], sB[FT_MAXX], sC[FT_MAXX];
-    __shared__ int sRed[4][4];
-    __shared__ int sBox[4];
-
-    const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    const int c = blockIdx.x * FT_COLS + lane;
-    const int rbase = blockIdx.y * FT_ROWS + wv * 4;
-
-    int iy[4], fy[4], ix[4], fx[4], reg[4];
-    int ymin = INT_MAX, ymax = INT_MIN, xmin = INT_MAX, xmax = INT_MIN;
+    float r = 0.f, i = 0.f;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        int r = rbase + k;
-        reg[k] = -1;
-        if (r < p.m_lines && c < p.m_pixels) {
-            slave_pos(p, r, c, iy[k], fy[k], ix[k], fx[k]);
-            reg[k] = region_of(p, iy[k], ix[k]);
-            if (reg[k] == 2) {
-                ymin = min(ymin, iy[k]);
-                ymax = max(ymax, iy[k]);
-                xmin = min(xmin, ix[k]);
-                xmax = max(xmax, ix[k]);
+    for (int j = 0; j < kTaps; j++) {
+        r = fmaf(kl[j], w[O + j].x, r);
+        i = fmaf(kl[j], w[O + j].y, i);
+    }
+    return make_float2(r, i);
+}
+
+// One strip (sx, sy) of one burst.  p may live in global memory (batched
+// launches): its fields are wave-uniform.
+struct WalkLds {
+    float2 u[WK_KEYS][64];      // azimuth sums per key and column
+    double pu[WK_KEYS][64];     // azimuth-interpolated phase per key and column
+    float2 win[WK_WIN][64];     // register windows, for rows served directly
+    double abc[3][64];          // per-column phase coefficients a, b, c
+};
+
+__device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, int sy,
+                                               int strip_rows, WalkLds& sm)
+{
+    const int lane = threadIdx.x;
+    const int c0 = sx * WK_OUT;
+    const int r0 = sy * strip_rows;
+    const int r1 = min(r0 + strip_rows, p.m_lines);
+    const int c = c0 + lane;
+    const bool out_lane = lane < WK_OUT && c < p.m_pixels;
+
+    // column terms of the two position polynomials: same operations, same
+    // order as slave_pos (contraction off), with the row and column
+    // products hoisted
+    double pn, t2a, t5a, t2r, t5r;
+    {
+#pragma clang fp contract(off)
+        pn = norm_warp((double)(p.mb[0] + min(c, p.m_pixels - 1)), (double)p.mb[0],
+                       (double)p.mb[1]);
+        t2a = p.cpm_az[2] * pn;
+        t5a = p.cpm_az[5] * pn * pn;
+        t2r = p.cpm_rg[2] * pn;
+        t5r = p.cpm_rg[5] * pn * pn;
+    }
+
+    // slave column of this lane
+    int iy0, fy0, ix0, fx0;
+    slave_pos(p, r0, c0, iy0, fy0, ix0, fx0);
+    // one column of slack on the left (and 2 on the right) for the slave
+    // column drifting along the strip; same value in every lane
+    const int ucol0 = ix0 - kHalf - 1;
+    ColumnWalker w;
+    w.xcol = ucol0 + lane;
+    w.in_box = w.xcol >= p.sb[0] && w.xcol <= p.sb[1];
+    w.a = w.b = w.c = 0.0;
+    if (w.in_box) phase_coeffs(p, w.xcol, w.a, w.b, w.c);
+    {
+        double s, cc;
+        double d = 2.0 * w.c;
+        sincos(fma(-rint(d * kInvTwoPi), kTwoPi, d), &s, &cc);
+        w.qr = cc; w.qi = s;
+    }
+
+    float2 win[WK_WIN];
+    int wbase = INT_MIN;   // absolute slave line held in win[0]
+
+    for (int r = r0; r < r1; r++) {
+            // ---- this lane's output position (slave_pos, hoisted form)
+        int iy = 0, fy = 0, ix = 0, fx = 0, reg = -1;
+        {
+#pragma clang fp contract(off)
+            double ln = norm_warp((double)(p.mb[2] + r), (double)p.mb[2], (double)p.mb[3]);
+            double ys = p.cpm_az[0] + p.cpm_az[1] * ln + t2a + p.cpm_az[3] * ln * ln +
+                        p.cpm_az[4] * ln * pn + t5a + p.az_shift;
+            double xs = p.cpm_rg[0] + p.cpm_rg[1] * ln + t2r + p.cpm_rg[3] * ln * ln +
+                        p.cpm_rg[4] * ln * pn + t5r;
+            ys = fmin(fmax(ys, -1.0e9), 1.0e9);
+            xs = fmin(fmax(xs, -1.0e9), 1.0e9);
+            iy = (int)trunc(ys);
+            fy = (int)((ys - iy) * kInterval + 0.5);
+            ix = (int)trunc(xs);
+            fx = (int)((xs - ix) * kInterval + 0.5);
+        }
+        if (out_lane) reg = region_of(p, iy, ix);
+        const int base = ix - kHalf - ucol0;
+        const bool servable = reg == 2 && base >= 0 && base <= 64 - kTaps;
+        const int key = servable ? iy * kLutRows + fy : INT_MAX;
+
+        // ---- distinct azimuth keys of the row, smallest first
+        int kcur = __builtin_amdgcn_readfirstlane(wave_min(key));
+        int mybuf = -1;
+        const bool has_window = kcur != INT_MAX;
+        if (has_window) {
+            const int iy_min = kcur / kLutRows;
+            // window: rows iy_min-5 .. iy_min+7 must be held
+            int o = wbase == INT_MIN ? -1 : iy_min - kHalf - wbase;
+            if (o < 0 || o > 64) {
+                wbase = iy_min - kHalf;
+                w.anchor(p, wbase);
+#pragma unroll
+                for (int k = 0; k < WK_WIN; k++) win[k] = w.next(p);
+                o = 0;
+            }
+            while (o > 3) {   // slide by 4 rows
+#pragma unroll
+                for (int k = 0; k < WK_WIN - 4; k++) win[k] = win[k + 4];
+#pragma unroll
+                for (int k = WK_WIN - 4; k < WK_WIN; k++) win[k] = w.next(p);
+                wbase += 4;
+                o -= 4;
+            }
+            int nk = 0;
+            while (kcur != INT_MAX && nk < WK_KEYS) {
+                const int iyq = kcur / kLutRows, fyq = kcur - iyq * kLutRows;
+                const int oq = o + (iyq - iy_min);
+                if (oq <= 4) {
+                    float kl[kTaps];
+                    load_taps(p.kaz, fyq, kl);
+                    float2 u;
+                    switch (oq) {
+                        case 0: u = az_dot<0>(win, kl); break;
+                        case 1: u = az_dot<1>(win, kl); break;
+                        case 2: u = az_dot<2>(win, kl); break;
+                        case 3: u = az_dot<3>(win, kl); break;
+                        default: u = az_dot<4>(win, kl); break;
+                    }
+                    double4 m = reinterpret_cast<const double4*>(p.mom)[fyq];
+                    double y0 = (double)(iyq - p.first_line);
+                    double bq = fma(y0, m.x, m.y);
+                    double cq = y0 * y0 * m.x + 2.0 * y0 * m.y + m.z;
+                    sm.u[nk][lane] = u;
+                    sm.pu[nk][lane] = fma(m.x, w.a, fma(bq, w.b, cq * w.c));
+                    if (key == kcur) mybuf = nk;
+                    nk++;
+                }
+                kcur = __builtin_amdgcn_readfirstlane(wave_min(key > kcur ? key : INT_MAX));
             }
         }
-    }
-    ymin = wave_min(ymin);
-    ymax = wave_max(ymax);
-    xmin = wave_min(xmin);
-    xmax = wave_max(xmax);
-    if (lane == 0) {
-        sRed[wv][0] = ymin; sRed[wv][1] = ymax; sRed[wv][2] = xmin; sRed[wv][3] = xmax;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int a = INT_MAX, b = INT_MIN, cc = INT_MAX, d = INT_MIN;
-        for (int w = 0; w < 4; w++) {
-            a = min(a, sRed[w][0]); b = max(b, sRed[w][1]);
-            cc = min(cc, sRed[w][2]); d = max(d, sRed[w][3]);
-        }
-        sBox[0] = a; sBox[1] = b; sBox[2] = cc; sBox[3] = d;
-    }
-    __syncthreads();
-    const bool any_interior = sBox[0] <= sBox[1];
-    const int Y0 = sBox[0] - kHalf, X0 = sBox[2] - kHalf;
-    const int ny = any_interior ? sBox[1] + kHalf + 1 - Y0 + 1 : 0;
-    const int nx = any_interior ? sBox[3] + kHalf + 1 - X0 + 1 : 0;
-    const bool staged = any_interior && ny <= FT_MAXY && nx <= FT_MAXX;
-
-    if (staged) {
-        // per-column phase coefficients, then the deramped window
-        for (int i = threadIdx.x; i < nx; i += 256) {
-            double a, b, cc;
-            phase_coeffs(p, X0 + i, a, b, cc);
-            sA[i] = a; sB[i] = b; sC[i] = cc;
-        }
         __syncthreads();
-        const int n = ny * nx;
-        for (int q = threadIdx.x; q < n; q += 256) {
-            int yy = q / nx, xx = q - yy * nx;
-            int ya = Y0 + yy, xa = X0 + xx;
-            double y = (double)(ya - p.first_line);
-            float sn, cs;
-            sincosf(reduce_2pi(fma(y, fma(y, sC[xx], sB[xx]), sA[xx])), &sn, &cs);
-            tops_ci16 s = p.slave[(size_t)(ya - p.sb[2]) * p.s_pixels + (xa - p.sb[0])];
-            float re = (float)s.re, im = (float)s.im;
-            sD[q] = make_float2(re * cs - im * sn, re * sn + im * cs);
-        }
-        __syncthreads();
-    }
 
-#pragma unroll 1
-    for (int k = 0; k < 4; k++) {
-        int r = rbase + k;
-        if (reg[k] < 0) continue;
-        tops_cf32 o = {0.0f, 0.0f};
-        if (reg[k] == 1) {
-            tops_ci16 s = p.slave[(size_t)(iy[k] - p.sb[2]) * p.s_pixels + (ix[k] - p.sb[0])];
-            o.x = (float)s.re;
-            o.y = (float)s.im;
-        } else if (reg[k] == 2) {
-            if (staged) {
-                float kl[kTaps], kp[kTaps];
-                load_taps(p.kaz, fy[k], kl);
-                load_taps(p.krg, fx[k], kp);
-                const int oy = iy[k] - kHalf - Y0, ox = ix[k] - kHalf - X0;
-                const float2* w = sD + oy * nx + ox;
-                float sr = 0.f, si = 0.f;
-#pragma unroll 2
-                for (int j = 0; j < kTaps; j++) {
-                    float tr = 0.f, ti = 0.f;
+        // ---- rows with more azimuth keys than the shared pass takes: the
+        // wave's register windows go to LDS and those outputs do their own
+        // 12x12 sum from there
+        const int orow = iy - kHalf - wbase;
+        const bool lds_direct = servable && mybuf < 0 && has_window && orow >= 0 && orow <= 4;
+        if (__ballot(lds_direct) != 0) {
 #pragma unroll
-                    for (int i = 0; i < kTaps; i++) {
-                        float2 d = w[j * nx + i];
-                        tr = fmaf(kp[i], d.x, tr);
-                        ti = fmaf(kp[i], d.y, ti);
-                    }
-                    sr = fmaf(kl[j], tr, sr);
-                    si = fmaf(kl[j], ti, si);
-                }
-                double A = 0.0, B = 0.0, C = 0.0;
+            for (int k = 0; k < WK_WIN; k++) sm.win[k][lane] = win[k];
+            sm.abc[0][lane] = w.a;
+            sm.abc[1][lane] = w.b;
+            sm.abc[2][lane] = w.c;
+            __syncthreads();
+        }
+
+        // ---- outputs
+        if (out_lane) {
+            tops_cf32 o = {0.0f, 0.0f};
+            if (reg == 1) {
+                tops_ci16 s = p.slave[(size_t)(iy - p.sb[2]) * p.s_pixels + (ix - p.sb[0])];
+                o.x = (float)s.re;
+                o.y = (float)s.im;
+            } else if (reg == 2 && mybuf >= 0) {
+                float kp[kTaps];
+                load_taps(p.krg, fx, kp);
+                const float2* u = &sm.u[mybuf][base];
+                const double* pu = &sm.pu[mybuf][base];
+                float sr = 0.f, si = 0.f;
+                double ph = 0.0;
 #pragma unroll
                 for (int i = 0; i < kTaps; i++) {
-                    double kk = (double)kp[i];
-                    A = fma(kk, sA[ox + i], A);
-                    B = fma(kk, sB[ox + i], B);
-                    C = fma(kk, sC[ox + i], C);
+                    float2 v = u[i];
+                    sr = fmaf(kp[i], v.x, sr);
+                    si = fmaf(kp[i], v.y, si);
+                    ph = fma((double)kp[i], pu[i], ph);
                 }
-                double4 m = reinterpret_cast<const double4*>(p.mom)[fy[k]];
-                double y0 = (double)(iy[k] - p.first_line);
-                double phi = m.x * A + fma(y0, m.x, m.y) * B +
-                             (y0 * y0 * m.x + 2.0 * y0 * m.y + m.z) * C;
-                o = reramp(sr, si, phi);
-            } else {
-                o = interior_direct(p, iy[k], fy[k], ix[k], fx[k]);
+                o = reramp(sr, si, ph);
+            } else if (lds_direct) {
+                // same operations, same order as the shared pass: azimuth
+                // sum per column, then the range taps over U and PU
+                float kl[kTaps], kp[kTaps];
+                load_taps(p.kaz, fy, kl);
+                load_taps(p.krg, fx, kp);
+                double4 m = reinterpret_cast<const double4*>(p.mom)[fy];
+                double y0 = (double)(iy - p.first_line);
+                double bq = fma(y0, m.x, m.y);
+                double cq = y0 * y0 * m.x + 2.0 * y0 * m.y + m.z;
+                float sr = 0.f, si = 0.f;
+                double ph = 0.0;
+#pragma unroll 2
+                for (int i = 0; i < kTaps; i++) {
+                    const int col = base + i;
+                    float ur = 0.f, ui = 0.f;
+#pragma unroll
+                    for (int j = 0; j < kTaps; j++) {
+                        float2 v = sm.win[orow + j][col];
+                        ur = fmaf(kl[j], v.x, ur);
+                        ui = fmaf(kl[j], v.y, ui);
+                    }
+                    sr = fmaf(kp[i], ur, sr);
+                    si = fmaf(kp[i], ui, si);
+                    double pu = fma(m.x, sm.abc[0][col], fma(bq, sm.abc[1][col], cq * sm.abc[2][col]));
+                    ph = fma((double)kp[i], pu, ph);
+                }
+                o = reramp(sr, si, ph);
+            } else if (reg == 2) {
+                o = interior_direct(p, iy, fy, ix, fx);
             }
+            p.out[(size_t)r * p.m_pixels + c] = o;
         }
-        p.out[(size_t)r * p.m_pixels + c] = o;
+        __syncthreads();   // LDS is rewritten by the next row
     }
 }
 
-int launch_resample_fast(const ResampleParams& p, hipStream_t s)
+__global__ void __launch_bounds__(64) resample_fast_kernel(ResampleParams p, int strip_rows)
 {
-    dim3 grid((p.m_pixels + FT_COLS - 1) / FT_COLS, (p.m_lines + FT_ROWS - 1) / FT_ROWS);
-    hipLaunchKernelGGL(resample_fast_kernel, grid, dim3(256), 0, s, p);
+    __shared__ WalkLds sm;
+    resample_strip(p, blockIdx.x, blockIdx.y, strip_rows, sm);
+}
+
+// Batched form: one workgroup per entry of a strip table covering many
+// bursts (work[i] = {burst, strip x, strip y, strip rows}).
+__global__ void __launch_bounds__(64) resample_batch_kernel(const ResampleParams* items,
+                                                            const int4* work)
+{
+    __shared__ WalkLds sm;
+    const int4 w = work[blockIdx.x];
+    resample_strip(items[w.x], w.y, w.z, w.w, sm);
+}
+
+int resample_strip_count(int m_lines, int m_pixels, int strip_rows, int* nx, int* ny)
+{
+    *nx = (m_pixels + WK_OUT - 1) / WK_OUT;
+    *ny = (m_lines + strip_rows - 1) / strip_rows;
+    return *nx * *ny;
+}
+
+int launch_resample_fast(const ResampleParams& p, hipStream_t s, int strip_rows)
+{
+    if (strip_rows <= 0) strip_rows = WK_ROWS;
+    dim3 grid((p.m_pixels + WK_OUT - 1) / WK_OUT, (p.m_lines + strip_rows - 1) / strip_rows);
+    hipLaunchKernelGGL(resample_fast_kernel, grid, dim3(64), 0, s, p, strip_rows);
+    TOPS_HIP(hipGetLastError());
+    return TOPS_OK;
+}
+
+int launch_resample_batch(const ResampleParams* d_items, const int4* d_work, int n_work,
+                          hipStream_t s)
+{
+    if (n_work <= 0) return TOPS_OK;
+    hipLaunchKernelGGL(resample_batch_kernel, dim3(n_work), dim3(64), 0, s, d_items, d_work);
     TOPS_HIP(hipGetLastError());
     return TOPS_OK;
 }

@@ -217,3 +217,405 @@ double tops_spectral_sep(const double* kt, int n_bursts, int spb, int lines_per_
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Per-subswath stages over in-memory bursts
+// ===========================================================================
+namespace {
+
+struct DBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    ~DBuf() { release(); }
+    int ensure(size_t bytes)
+    {
+        if (bytes <= n) return TOPS_OK;
+        release();
+        TOPS_HIP(hipMalloc(&p, bytes ? bytes : 1));
+        n = bytes;
+        return TOPS_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+int check_swath(const tops_swath* s, const char* who)
+{
+    if (!s || !s->slc || !s->burst_first_line_time || !s->burst_last_line_time ||
+        s->lines_per_burst <= 0 || s->samples_per_burst <= 0 || s->n_bursts <= 0)
+        return set_error(TOPS_ERR_ARG, "%s: incomplete swath description", who);
+    return TOPS_OK;
+}
+
+int check_range(const tops_swath* m, const tops_swath* s, int mb0, int mbN, int sb0, int sbN,
+                const char* who)
+{
+    if (mb0 < 0 || mbN < mb0 || mbN >= m->n_bursts || sb0 < 0 || sbN >= s->n_bursts ||
+        sbN - sb0 != mbN - mb0)
+        return set_error(TOPS_ERR_ARG, "%s: burst ranges [%d,%d] / [%d,%d] do not match the swaths",
+                         who, mb0, mbN, sb0, sbN);
+    return TOPS_OK;
+}
+
+// RefDem::getData (DataTypes.cpp:240-300): window around the lat/lon box
+// extended by `margin` posts, clipped to the DEM; returns its origin.
+bool dem_window(const tops_dem* d, double lat_min, double lat_max, double lon_min,
+                double lon_max, int margin, int* r0, int* c0, int* lines, int* pixels)
+{
+    double ul0 = floor((d->lat_max - (lat_max + margin * d->dlat)) / d->dlat);
+    double ul1 = floor(((lon_min - margin * d->dlon) - d->lon_min) / d->dlon);
+    double lr0 = ceil((d->lat_max - (lat_min - margin * d->dlat)) / d->dlat);
+    double lr1 = ceil(((lon_max + margin * d->dlon) - d->lon_min) / d->dlon);
+    if (ul0 < 0) ul0 = 0;
+    if (ul1 < 0) ul1 = 0;
+    if (lr0 > d->lines - 1) lr0 = d->lines - 1;
+    if (lr1 > d->pixels - 1) lr1 = d->pixels - 1;
+    if (lr0 < ul0 || lr1 < ul1) return false;
+    *r0 = (int)ul0;
+    *c0 = (int)ul1;
+    *lines = (int)(lr0 - ul0) + 1;
+    *pixels = (int)(lr1 - ul1) + 1;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// GeometricCoreg, Func.cpp:2215-2297.  For each master burst b of
+// [mburst0, mburstN] (slave burst b + sburst0 - mburst0): box x in
+// [0, spb-1], y in [b lpb, (b+1) lpb - 1]; DEM window = the burst's lat/lon
+// box plus 24 posts; Newton start = mid-burst time.  Differences from the
+// reference, which has these index bugs: the slave uses its own burst's
+// first-line and mid times (the reference indexes them with the master burst,
+// Func.cpp:2235, 2274), the DEM origin passed on is the window's own
+// (Func.cpp:2273 passes the burst's lat_max/lon_min), and the whole int32
+// window is used (Geocoreg_cuda.cu:788 copies half of it).
+// cpm_az/cpm_rg: [(mburstN - mburst0 + 1) x 6].  A burst whose fit fails
+// gets zero coefficients; the call then returns TOPS_ERR_NOT_PD after
+// processing the other bursts.
+int tops_geometric_coreg(const tops_swath* m, const tops_swath* s, const tops_orbit* mo,
+                         const tops_orbit* so, const tops_dem* dem, int mburst0, int mburstN,
+                         int sburst0, int sburstN, double* cpm_az, double* cpm_rg)
+{
+    int rc = check_swath(m, "tops_geometric_coreg");
+    if (!rc) rc = check_swath(s, "tops_geometric_coreg");
+    if (!rc) rc = check_range(m, s, mburst0, mburstN, sburst0, sburstN, "tops_geometric_coreg");
+    if (rc) return rc;
+    if (!mo || !so || !dem || !dem->data || !cpm_az || !cpm_rg || !m->burst_ll)
+        return set_error(TOPS_ERR_ARG, "tops_geometric_coreg: null argument");
+    const int gap = sburst0 - mburst0;
+    DBuf d_dem;
+    int status = TOPS_OK;
+    for (int b = mburst0; b <= mburstN; b++) {
+        const int sbi = b + gap;
+        const double* ll = m->burst_ll + 4 * b;
+        int r0, c0, lines, pixels;
+        double* ca = cpm_az + 6 * (b - mburst0);
+        double* cr = cpm_rg + 6 * (b - mburst0);
+        if (!dem_window(dem, ll[0], ll[1], ll[2], ll[3], 24, &r0, &c0, &lines, &pixels)) {
+            for (int i = 0; i < 6; i++) ca[i] = cr[i] = 0.0;
+            status = set_error(TOPS_ERR_NOT_PD, "burst %d: no DEM coverage", b);
+            continue;
+        }
+        std::vector<int32_t> win((size_t)lines * pixels);
+        for (int i = 0; i < lines; i++)
+            memcpy(&win[(size_t)i * pixels], dem->data + (size_t)(r0 + i) * dem->pixels + c0,
+                   sizeof(int32_t) * pixels);
+        if ((rc = d_dem.ensure(win.size() * sizeof(int32_t)))) return rc;
+        TOPS_HIP(hipMemcpy(d_dem.p, win.data(), win.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        tops_orbit_geom gm = {mo->coef_x, mo->coef_y, mo->coef_z, mo->ncoef,
+                              mo->az_time[mo->num_points / 2], m->az_time_interval,
+                              m->burst_first_line_time[b], m->slr_time_first_pixel,
+                              m->range_spacing,
+                              0.5 * (m->burst_first_line_time[b] + m->burst_last_line_time[b]),
+                              b * m->lines_per_burst};
+        tops_orbit_geom gs = {so->coef_x, so->coef_y, so->coef_z, so->ncoef,
+                              so->az_time[so->num_points / 2], s->az_time_interval,
+                              s->burst_first_line_time[sbi], s->slr_time_first_pixel,
+                              s->range_spacing,
+                              0.5 * (s->burst_first_line_time[sbi] + s->burst_last_line_time[sbi]),
+                              sbi * s->lines_per_burst};
+        rc = tops_geocoreg(d_dem.as<int32_t>(), lines, pixels, dem->lat_max - r0 * dem->dlat,
+                           dem->lon_min + c0 * dem->dlon, dem->dlat, dem->dlon, dem->nodata, &gm,
+                           &gs, 0, m->samples_per_burst - 1, b * m->lines_per_burst,
+                           (b + 1) * m->lines_per_burst - 1, ca, cr, nullptr, nullptr, nullptr);
+        if (rc == TOPS_ERR_NOT_PD) {
+            status = rc;
+            continue;
+        }
+        if (rc) return rc;
+    }
+    return status;
+}
+
+// DerampAndResample, Func.cpp:2326-2406: each slave burst b + gap is
+// deramped, resampled onto master burst b and reramped; out is the
+// burst-stacked [(mburstN - mburst0 + 1) * lpb_m x spb_m] result (the
+// reference's ReSlave raster).  The per-range arrays of the slave burst
+// itself are used (the reference always passes the burst-0 row,
+// Func.cpp:2383-2384).  d_out, when not NULL, receives the same raster on
+// the device (dense) and out may then be NULL.
+int tops_deramp_and_resample(const tops_swath* m, const tops_swath* s, const double* cpm_az,
+                             const double* cpm_rg, int mburst0, int mburstN, int sburst0,
+                             int sburstN, const float* kernel_az, const float* kernel_rg,
+                             tops_cf32* out, tops_cf32* d_out)
+{
+    int rc = check_swath(m, "tops_deramp_and_resample");
+    if (!rc) rc = check_swath(s, "tops_deramp_and_resample");
+    if (!rc) rc = check_range(m, s, mburst0, mburstN, sburst0, sburstN, "tops_deramp_and_resample");
+    if (rc) return rc;
+    if (!cpm_az || !cpm_rg || !kernel_az || !kernel_rg || (!out && !d_out) ||
+        !s->doppler_rate || !s->reference_time || !s->doppler_centroid)
+        return set_error(TOPS_ERR_ARG, "tops_deramp_and_resample: null argument");
+    tops_ctx* ctx = tops_ctx_create(kernel_az, kernel_rg, kTaps);
+    if (!ctx) return tops_last_error();
+    const int lpm = m->lines_per_burst, spm = m->samples_per_burst;
+    const int lps = s->lines_per_burst, sps = s->samples_per_burst;
+    const size_t mbytes = (size_t)lpm * spm * sizeof(tops_cf32);
+    DBuf d_s, d_par, d_o;
+    rc = d_s.ensure((size_t)lps * sps * sizeof(tops_ci16));
+    if (!rc) rc = d_par.ensure(3 * sizeof(double) * sps);
+    if (!rc && !d_out) rc = d_o.ensure(mbytes);
+    for (int b = mburst0; b <= mburstN && !rc; b++) {
+        const int sbi = b + sburst0 - mburst0;
+        hipError_t e = hipMemcpy(d_s.p, s->slc + (size_t)sbi * lps * sps,
+                                 (size_t)lps * sps * sizeof(tops_ci16), hipMemcpyHostToDevice);
+        double* par = d_par.as<double>();
+        if (e == hipSuccess)
+            e = hipMemcpy(par, s->doppler_rate + (size_t)sbi * sps, sizeof(double) * sps,
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(par + sps, s->reference_time + (size_t)sbi * sps, sizeof(double) * sps,
+                          hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(par + 2 * sps, s->doppler_centroid + (size_t)sbi * sps,
+                          sizeof(double) * sps, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "tops_deramp_and_resample upload", __FILE__, __LINE__);
+            break;
+        }
+        tops_burst_geom g;
+        int mbox[4] = {0, spm - 1, b * lpm, (b + 1) * lpm - 1};
+        int sbox[4] = {0, sps - 1, sbi * lps, (sbi + 1) * lps - 1};
+        memcpy(g.master_box, mbox, sizeof(mbox));
+        memcpy(g.slave_box, sbox, sizeof(sbox));
+        g.first_line_in_burst = sbi * lps;
+        g.samples_per_burst = sps;
+        g.az_time_interval = s->az_time_interval;
+        g.az_shift = 0.0;
+        memcpy(g.cpm_az, cpm_az + 6 * (b - mburst0), sizeof(g.cpm_az));
+        memcpy(g.cpm_rg, cpm_rg + 6 * (b - mburst0), sizeof(g.cpm_rg));
+        tops_cf32* dst = d_out ? d_out + (size_t)(b - mburst0) * lpm * spm : d_o.as<tops_cf32>();
+        rc = tops_resample_burst(ctx, &g, d_s.as<tops_ci16>(), par, par + sps, par + 2 * sps, dst,
+                                 TOPS_MODE_FAST, nullptr);
+        if (!rc && out) {
+            e = hipMemcpy(out + (size_t)(b - mburst0) * lpm * spm, dst, mbytes, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = hip_fail(e, "tops_deramp_and_resample readback", __FILE__, __LINE__);
+        }
+    }
+    if (!rc) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = hip_fail(e, "tops_deramp_and_resample", __FILE__, __LINE__);
+    }
+    tops_ctx_destroy(ctx);
+    return rc;
+}
+
+// ESDAndCoh, Func.cpp:2496-2763, over the pass-1 result `reslave`
+// ([(mburstN - mburst0 + 1) * lpb x spb], as tops_deramp_and_resample
+// writes it):
+//   1. overlap sizes of the master bursts (EstOverlapSize) and the spectral
+//      separation of the master swath (GetSpectralSep);
+//   2. forward/backward bands of the master SLC and of reslave, ESD shifts
+//      with window esd_win_az x esd_win_rg and threshold (the reference
+//      hard-codes 6 x 24 and 0.3, Func.cpp:2583-2585); |shift| > 100 -> 0;
+//   3. burst mburst0 keeps its pass-1 result; every later burst b is
+//      resampled again with AzimuthShift = shift[b - mburst0 - 1] (not a
+//      cumulative shift, as the reference, Func.cpp:2702);
+//   4. coherence of every burst against the master, window coh_win_az x
+//      coh_win_rg starting (win-1)/2 before the sample.
+// Outputs (host): reslave_esd [N lpb x spb], coh [N lpb x spb], shifts
+// [mburstN - mburst0] (any may be NULL except shifts when N > 1).
+int tops_esd_and_coh(const tops_swath* m, const tops_swath* s, const double* cpm_az,
+                     const double* cpm_rg, int mburst0, int mburstN, int sburst0, int sburstN,
+                     const float* kernel_az, const float* kernel_rg, const tops_cf32* reslave,
+                     int esd_win_az, int esd_win_rg, float threshold, int coh_win_az,
+                     int coh_win_rg, tops_cf32* reslave_esd, float* coh, double* shifts)
+{
+    int rc = check_swath(m, "tops_esd_and_coh");
+    if (!rc) rc = check_swath(s, "tops_esd_and_coh");
+    if (!rc) rc = check_range(m, s, mburst0, mburstN, sburst0, sburstN, "tops_esd_and_coh");
+    if (rc) return rc;
+    const int nb = mburstN - mburst0 + 1, nov = nb - 1;
+    if (!cpm_az || !cpm_rg || !kernel_az || !kernel_rg || !reslave || (nov > 0 && !shifts) ||
+        !m->doppler_rate)
+        return set_error(TOPS_ERR_ARG, "tops_esd_and_coh: null argument");
+    const int lpb = m->lines_per_burst, spb = m->samples_per_burst;
+    const size_t bpix = (size_t)lpb * spb;
+    std::vector<double> sh(nov > 0 ? nov : 1, 0.0);
+    if (nov > 0) {
+        std::vector<int> sizes(nov);
+        int total = 0, maximum = 0;
+        tops_overlap_sizes(m->burst_first_line_time, m->burst_last_line_time, mburst0, mburstN,
+                           m->az_time_interval, sizes.data(), &total, &maximum);
+        for (int i = 0; i < nov; i++)
+            if (sizes[i] < 0 || sizes[i] > lpb)
+                return set_error(TOPS_ERR_ARG, "tops_esd_and_coh: overlap %d has %d lines", i,
+                                 sizes[i]);
+        double df = tops_spectral_sep(m->doppler_rate, m->n_bursts, spb, lpb, m->az_time_interval);
+        std::vector<tops_ci16> mf((size_t)total * spb), mbk((size_t)total * spb);
+        std::vector<tops_cf32> sf((size_t)total * spb), sbk((size_t)total * spb);
+        size_t off = 0;
+        for (int i = 0; i < nov; i++) {   // DataTypes.cpp:456-513 band gather
+            const int n = sizes[i];
+            const size_t fwd = (size_t)(i + 1) * lpb - n, bwd = (size_t)(i + 1) * lpb;
+            memcpy(&mf[off * spb], m->slc + ((size_t)mburst0 * lpb + fwd) * spb,
+                   sizeof(tops_ci16) * n * spb);
+            memcpy(&mbk[off * spb], m->slc + ((size_t)mburst0 * lpb + bwd) * spb,
+                   sizeof(tops_ci16) * n * spb);
+            memcpy(&sf[off * spb], reslave + fwd * spb, sizeof(tops_cf32) * n * spb);
+            memcpy(&sbk[off * spb], reslave + bwd * spb, sizeof(tops_cf32) * n * spb);
+            off += n;
+        }
+        DBuf a, b2, c, d;
+        size_t n = (size_t)total * spb;
+        if ((rc = a.ensure(n * 4)) || (rc = b2.ensure(n * 4)) || (rc = c.ensure(n * 8)) ||
+            (rc = d.ensure(n * 8)))
+            return rc;
+        TOPS_HIP(hipMemcpy(a.p, mf.data(), n * 4, hipMemcpyHostToDevice));
+        TOPS_HIP(hipMemcpy(b2.p, mbk.data(), n * 4, hipMemcpyHostToDevice));
+        TOPS_HIP(hipMemcpy(c.p, sf.data(), n * 8, hipMemcpyHostToDevice));
+        TOPS_HIP(hipMemcpy(d.p, sbk.data(), n * 8, hipMemcpyHostToDevice));
+        rc = tops_esd_shifts(a.as<tops_ci16>(), b2.as<tops_ci16>(), c.as<tops_cf32>(),
+                             d.as<tops_cf32>(), sizes.data(), nov, spb, esd_win_az, esd_win_rg,
+                             threshold, df, m->az_time_interval, sh.data(), nullptr, nullptr);
+        if (rc) return rc;
+        for (int i = 0; i < nov; i++) {
+            if (fabs(sh[i]) > 100) sh[i] = 0;   // Func.cpp:2612-2621
+            shifts[i] = sh[i];
+        }
+    }
+    // pass 2: re-resample with the shift, then coherence
+    tops_ctx* ctx = tops_ctx_create(kernel_az, kernel_rg, kTaps);
+    if (!ctx) return tops_last_error();
+    const int lps = s->lines_per_burst, sps = s->samples_per_burst;
+    DBuf d_s, d_par, d_res, d_m, d_coh;
+    rc = d_s.ensure((size_t)lps * sps * sizeof(tops_ci16));
+    if (!rc) rc = d_par.ensure(3 * sizeof(double) * sps);
+    if (!rc) rc = d_res.ensure(bpix * sizeof(tops_cf32));
+    if (!rc) rc = d_m.ensure(bpix * sizeof(tops_ci16));
+    if (!rc) rc = d_coh.ensure(bpix * sizeof(float));
+    for (int b = mburst0; b <= mburstN && !rc; b++) {
+        const int k = b - mburst0, sbi = b + sburst0 - mburst0;
+        hipError_t e = hipSuccess;
+        if (k == 0) {
+            e = hipMemcpy(d_res.p, reslave, bpix * sizeof(tops_cf32), hipMemcpyHostToDevice);
+        } else {
+            e = hipMemcpy(d_s.p, s->slc + (size_t)sbi * lps * sps,
+                          (size_t)lps * sps * sizeof(tops_ci16), hipMemcpyHostToDevice);
+            double* par = d_par.as<double>();
+            if (e == hipSuccess)
+                e = hipMemcpy(par, s->doppler_rate + (size_t)sbi * sps, sizeof(double) * sps,
+                              hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(par + sps, s->reference_time + (size_t)sbi * sps,
+                              sizeof(double) * sps, hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(par + 2 * sps, s->doppler_centroid + (size_t)sbi * sps,
+                              sizeof(double) * sps, hipMemcpyHostToDevice);
+            if (e == hipSuccess) {
+                tops_burst_geom g;
+                int mbox[4] = {0, spb - 1, b * lpb, (b + 1) * lpb - 1};
+                int sbox[4] = {0, sps - 1, sbi * lps, (sbi + 1) * lps - 1};
+                memcpy(g.master_box, mbox, sizeof(mbox));
+                memcpy(g.slave_box, sbox, sizeof(sbox));
+                // Func.cpp:2701-2705 passes the master's lines per burst and
+                // azimuth interval here
+                g.first_line_in_burst = sbi * lpb;
+                g.samples_per_burst = sps;
+                g.az_time_interval = m->az_time_interval;
+                g.az_shift = sh[k - 1];
+                memcpy(g.cpm_az, cpm_az + 6 * k, sizeof(g.cpm_az));
+                memcpy(g.cpm_rg, cpm_rg + 6 * k, sizeof(g.cpm_rg));
+                rc = tops_resample_burst(ctx, &g, d_s.as<tops_ci16>(), par, par + sps,
+                                         par + 2 * sps, d_res.as<tops_cf32>(), TOPS_MODE_FAST,
+                                         nullptr);
+            }
+        }
+        if (e != hipSuccess) rc = hip_fail(e, "tops_esd_and_coh upload", __FILE__, __LINE__);
+        if (rc) break;
+        if (reslave_esd) {
+            e = hipMemcpy(reslave_esd + (size_t)k * bpix, d_res.p, bpix * sizeof(tops_cf32),
+                          hipMemcpyDeviceToHost);
+            if (e != hipSuccess) { rc = hip_fail(e, "readback", __FILE__, __LINE__); break; }
+        }
+        if (coh) {
+            e = hipMemcpy(d_m.p, m->slc + (size_t)b * bpix, bpix * sizeof(tops_ci16),
+                          hipMemcpyHostToDevice);
+            if (e != hipSuccess) { rc = hip_fail(e, "upload", __FILE__, __LINE__); break; }
+            rc = tops_coherence(d_m.as<tops_ci16>(), d_res.as<tops_cf32>(), spb, lpb,
+                                (coh_win_rg - 1) / 2, (coh_win_az - 1) / 2, coh_win_rg, coh_win_az,
+                                d_coh.as<float>(), TOPS_MODE_FAST, nullptr);
+            if (rc) break;
+            e = hipMemcpy(coh + (size_t)k * bpix, d_coh.p, bpix * sizeof(float), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) { rc = hip_fail(e, "readback", __FILE__, __LINE__); break; }
+        }
+    }
+    tops_ctx_destroy(ctx);
+    return rc;
+}
+
+}  // extern "C"
+
+// ReadInput, Func.cpp:7-130: "key=value" (or key<TAB>value) lines with the
+// keys of ConfigSet; surrounding blanks are trimmed.  Returns TOPS_ERR_ARG
+// when the file cannot be opened.
+extern "C" int tops_read_config(const char* path, tops_config* cfg)
+{
+    if (!path || !cfg) return set_error(TOPS_ERR_ARG, "tops_read_config: null argument");
+    FILE* f = fopen(path, "r");
+    if (!f) return set_error(TOPS_ERR_ARG, "tops_read_config: cannot open %s", path);
+    memset(cfg, 0, sizeof(*cfg));
+    char line[2048];
+    while (fgets(line, sizeof(line), f)) {
+        char* k = line;
+        while (*k == ' ' || *k == '\t' || *k == '=') k++;
+        char* e = k;
+        while (*e && *e != '=' && *e != '\t' && *e != '\n' && *e != '\r') e++;
+        char* v = e;
+        while (*v == '=' || *v == '\t' || *v == ' ') v++;
+        char* ke = e;
+        while (ke > k && ke[-1] == ' ') ke--;
+        *ke = 0;
+        char* ve = v + strlen(v);
+        while (ve > v && (ve[-1] == '\n' || ve[-1] == '\r' || ve[-1] == ' ' || ve[-1] == '\t')) ve--;
+        *ve = 0;
+        struct { const char* key; char* dst; size_t n; } strs[] = {
+            {"process_dir", cfg->process_dir, sizeof(cfg->process_dir)},
+            {"masterpath", cfg->masterpath, sizeof(cfg->masterpath)},
+            {"slavepath", cfg->slavepath, sizeof(cfg->slavepath)},
+            {"polarisation", cfg->polarisation, sizeof(cfg->polarisation)},
+            {"preciseOrbitMaster", cfg->precise_orbit_master, sizeof(cfg->precise_orbit_master)},
+            {"preciseOrbitSlave", cfg->precise_orbit_slave, sizeof(cfg->precise_orbit_slave)},
+            {"SpecificDemPath", cfg->specific_dem_path, sizeof(cfg->specific_dem_path)}};
+        struct { const char* key; int* dst; } ints[] = {
+            {"firstsubswath", &cfg->firstsubswath}, {"lastsubswath", &cfg->lastsubswath},
+            {"multi_az", &cfg->multi_az}, {"multi_rg", &cfg->multi_rg},
+            {"burst0", &cfg->burst0}, {"burstN", &cfg->burstN}};
+        for (auto& sdef : strs)
+            if (!strcmp(k, sdef.key)) {
+                strncpy(sdef.dst, v, sdef.n - 1);
+                sdef.dst[sdef.n - 1] = 0;
+            }
+        for (auto& idef : ints)
+            if (!strcmp(k, idef.key)) *idef.dst = atoi(v);
+    }
+    fclose(f);
+    return TOPS_OK;
+}

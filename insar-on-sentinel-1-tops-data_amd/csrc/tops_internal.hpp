@@ -90,12 +90,25 @@ struct ResampleParams {
 
 // launchers (resample.hip)
 int launch_resample_fp64(tops_ctx* ctx, const ResampleParams& p, hipStream_t s);
-int launch_resample_fast(const ResampleParams& p, hipStream_t s);
+int launch_resample_fast(const ResampleParams& p, hipStream_t s, int strip_rows = 0);
+int launch_resample_batch(const ResampleParams* d_items, const int4* d_work, int n_work,
+                          hipStream_t s);
+int resample_strip_count(int m_lines, int m_pixels, int strip_rows, int* nx, int* ny);
+constexpr int kDefaultStripRows = 128;
 
 // launchers (coherence.hip)
 int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h,
                      int dx, int dy, int win_rg, int win_az, float* coh,
                      int mode, hipStream_t st);
+struct CohItem {
+    const tops_ci16* m;
+    const tops_cf32* s;
+    float* coh;
+    int w, h;
+};
+int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_work, int dx,
+                           int dy, int win_rg, int win_az, hipStream_t st);
+int coherence_tile_count(int w, int h, int* nx, int* ny);
 
 // launchers (esd.hip): writes 2*n_overlap double sums to d_sums
 int launch_esd(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* sf,

@@ -27,7 +27,10 @@ EXPORTS = [
     "tops_ctx_create", "tops_ctx_destroy", "tops_resample_burst",
     "tops_coherence", "tops_esd_shifts", "tops_geocoreg",
     "tops_lut_init", "tops_polyfit_orbit", "tops_orbit_speed", "tops_deramp_params",
-    "tops_overlap_sizes", "tops_spectral_sep",
+    "tops_overlap_sizes", "tops_spectral_sep", "tops_geometric_coreg",
+    "tops_deramp_and_resample", "tops_esd_and_coh", "tops_read_config",
+    "tops_plan_create", "tops_plan_run", "tops_plan_capture_graph", "tops_plan_tune",
+    "tops_plan_strip_rows", "tops_plan_destroy",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -49,6 +52,40 @@ class OrbitGeom(C.Structure):
                 ("first_line_time", C.c_double), ("slr_time_first_pixel", C.c_double),
                 ("range_spacing", C.c_double), ("az_init", C.c_double),
                 ("line_offset", C.c_int)]
+
+
+class BurstItem(C.Structure):
+    _fields_ = [("geom", BurstGeom), ("slave", _vp), ("kt", _vp), ("tref", _vp), ("fdc", _vp),
+                ("out", _vp), ("master", _vp), ("coh", _vp)]
+
+
+class Swath(C.Structure):
+    _fields_ = [("lines_per_burst", C.c_int), ("samples_per_burst", C.c_int),
+                ("n_bursts", C.c_int), ("az_time_interval", C.c_double),
+                ("range_spacing", C.c_double), ("slr_time_first_pixel", C.c_double),
+                ("wavelength", C.c_double), ("burst_first_line_time", _dp),
+                ("burst_last_line_time", _dp), ("burst_ll", _dp), ("doppler_rate", _dp),
+                ("reference_time", _dp), ("doppler_centroid", _dp), ("slc", _vp)]
+
+
+class Orbit(C.Structure):
+    _fields_ = [("num_points", C.c_int), ("az_time", _dp), ("ncoef", C.c_int),
+                ("coef_x", _dp), ("coef_y", _dp), ("coef_z", _dp)]
+
+
+class Dem(C.Structure):
+    _fields_ = [("data", _vp), ("lines", C.c_int), ("pixels", C.c_int), ("lat_max", C.c_double),
+                ("lon_min", C.c_double), ("dlat", C.c_double), ("dlon", C.c_double),
+                ("nodata", C.c_int)]
+
+
+class Config(C.Structure):
+    _fields_ = [("process_dir", C.c_char * 512), ("masterpath", C.c_char * 512),
+                ("slavepath", C.c_char * 512), ("polarisation", C.c_char * 32),
+                ("precise_orbit_master", C.c_char * 512), ("precise_orbit_slave", C.c_char * 512),
+                ("specific_dem_path", C.c_char * 512), ("firstsubswath", C.c_int),
+                ("lastsubswath", C.c_int), ("multi_az", C.c_int), ("multi_rg", C.c_int),
+                ("burst0", C.c_int), ("burstN", C.c_int)]
 
 
 _lib = None
@@ -121,6 +158,32 @@ def load():
     L.tops_overlap_sizes.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_double, _ip, _ip, _ip]
     L.tops_spectral_sep.restype = C.c_double
     L.tops_spectral_sep.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double]
+    L.tops_geometric_coreg.restype = C.c_int
+    L.tops_geometric_coreg.argtypes = [C.POINTER(Swath), C.POINTER(Swath), C.POINTER(Orbit),
+                                       C.POINTER(Orbit), C.POINTER(Dem), C.c_int, C.c_int,
+                                       C.c_int, C.c_int, _dp, _dp]
+    L.tops_deramp_and_resample.restype = C.c_int
+    L.tops_deramp_and_resample.argtypes = [C.POINTER(Swath), C.POINTER(Swath), _dp, _dp,
+                                           C.c_int, C.c_int, C.c_int, C.c_int, _fp, _fp, _vp,
+                                           _vp]
+    L.tops_esd_and_coh.restype = C.c_int
+    L.tops_esd_and_coh.argtypes = [C.POINTER(Swath), C.POINTER(Swath), _dp, _dp, C.c_int,
+                                   C.c_int, C.c_int, C.c_int, _fp, _fp, _vp, C.c_int, C.c_int,
+                                   C.c_float, C.c_int, C.c_int, _vp, _fp, _dp]
+    L.tops_plan_create.restype = _vp
+    L.tops_plan_create.argtypes = [_vp, C.POINTER(BurstItem), C.c_int, C.c_int, C.c_int, C.c_int]
+    L.tops_plan_run.restype = C.c_int
+    L.tops_plan_run.argtypes = [_vp, _vp]
+    L.tops_plan_capture_graph.restype = C.c_int
+    L.tops_plan_capture_graph.argtypes = [_vp, _vp]
+    L.tops_plan_tune.restype = C.c_int
+    L.tops_plan_tune.argtypes = [_vp, _vp, _ip]
+    L.tops_plan_strip_rows.restype = C.c_int
+    L.tops_plan_strip_rows.argtypes = [_vp]
+    L.tops_plan_destroy.restype = None
+    L.tops_plan_destroy.argtypes = [_vp]
+    L.tops_read_config.restype = C.c_int
+    L.tops_read_config.argtypes = [C.c_char_p, C.POINTER(Config)]
     _lib = L
     return L
 

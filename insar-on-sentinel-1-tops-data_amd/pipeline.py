@@ -1,0 +1,122 @@
+"""Stage-layer driver over a synthetic pair: what Main.cpp:84-134 does for one
+subswath (GeometricCoreg -> DerampAndResample -> ESDAndCoh), through the
+library's tops_geometric_coreg / tops_deramp_and_resample /
+tops_esd_and_coh, with the metadata steps of Main.cpp:57-76 (Polyfit_Orbit,
+ComputeDerampPara, InKernelInit) done by the library's host functions.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import native
+
+
+class SubswathRun:
+    """Inputs of one subswath pair, laid out as the stage functions read
+    them (host memory), plus the three stage calls."""
+
+    def __init__(self, pair, dem_margin_deg=0.02):
+        N = native
+        self.pair = pair
+        m, s = pair.master, pair.slave
+        self.m, self.s = m, s
+        self.nb = m.n_bursts
+        self.lpb, self.spb = m.lines_per_burst, m.samples_per_burst
+        # SLC rasters (burst-stacked)
+        self.m_slc = np.concatenate([pair.burst(b, "master") for b in range(self.nb)])
+        self.s_slc = np.concatenate([pair.burst(b, "slave") for b in range(self.nb)])
+        # orbits: Polyfit_Orbit
+        from .synth import orbit_polyfit_inputs
+        self.orbits = {}
+        for key, orbit, sw in (("m", pair.morbit, m), ("s", pair.sorbit, s)):
+            t, x, y, z = orbit_polyfit_inputs(orbit)
+            cx, cy, cz = N.polyfit_orbit(t, x, y, z)
+            speed = N.load().tops_orbit_speed(0.5 * (sw.first_line_time + sw.last_line_time),
+                                              t[len(t) // 2], N.ptr(cx), N.ptr(cy), N.ptr(cz),
+                                              len(cx))
+            kt, tref, fdc, ka = N.deramp_params(sw, speed)
+            self.orbits[key] = dict(t=np.ascontiguousarray(t), cx=cx, cy=cy, cz=cz, kt=kt,
+                                    tref=tref, fdc=fdc, speed=speed)
+        # InKernelInit with the master's PRF / ABW / RSR (Main.cpp:75-76)
+        self.kaz, self.kry = N.lut_init(12, m.prf, m.azimuth_bandwidth,
+                                        2 * m.range_sampling_rate)
+        # DEM covering every burst
+        ll = m.burst_ll
+        dem, lat0, lon0, dlat, dlon = pair.dem_window(ll[:, 0].min() - dem_margin_deg,
+                                                      ll[:, 1].max() + dem_margin_deg,
+                                                      ll[:, 2].min() - dem_margin_deg,
+                                                      ll[:, 3].max() + dem_margin_deg)
+        self.dem = np.ascontiguousarray(dem)
+        self.dem_geo = (lat0, lon0, dlat, dlon)
+        self._keep = []
+
+    # ---- ctypes structures ---------------------------------------------
+    def swath(self, which):
+        N = native
+        sw = self.m if which == "m" else self.s
+        o = self.orbits[which]
+        slc = self.m_slc if which == "m" else self.s_slc
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                (sw.burst_first_line_time, sw.burst_last_line_time, sw.burst_ll, o["kt"],
+                 o["tref"], o["fdc"])]
+        self._keep.append(arrs)
+        return N.Swath(sw.lines_per_burst, sw.samples_per_burst, sw.n_bursts,
+                       sw.az_time_interval, sw.range_spacing, sw.slr_time_first_pixel,
+                       sw.wavelength, *[N.ptr(a) for a in arrs], slc.ctypes.data)
+
+    def orbit(self, which):
+        N = native
+        o = self.orbits[which]
+        return N.Orbit(len(o["t"]), N.ptr(o["t"]), len(o["cx"]), N.ptr(o["cx"]), N.ptr(o["cy"]),
+                       N.ptr(o["cz"]))
+
+    def dem_struct(self):
+        lat0, lon0, dlat, dlon = self.dem_geo
+        return native.Dem(self.dem.ctypes.data, self.dem.shape[0], self.dem.shape[1], lat0, lon0,
+                          dlat, dlon, -32768)
+
+    # ---- stages ------------------------------------------------------------
+    def geometric_coreg(self, b0=0, bn=None):
+        N = native
+        bn = self.nb - 1 if bn is None else bn
+        n = bn - b0 + 1
+        ca, cr = np.zeros((n, 6)), np.zeros((n, 6))
+        ms, ss = self.swath("m"), self.swath("s")
+        mo, so = self.orbit("m"), self.orbit("s")
+        dem = self.dem_struct()
+        rc = N.load().tops_geometric_coreg(C.byref(ms), C.byref(ss), C.byref(mo), C.byref(so),
+                                           C.byref(dem), b0, bn, b0, bn, N.ptr(ca), N.ptr(cr))
+        N.check(rc, "tops_geometric_coreg")
+        return ca, cr
+
+    def deramp_and_resample(self, ca, cr, b0=0, bn=None):
+        N = native
+        bn = self.nb - 1 if bn is None else bn
+        n = bn - b0 + 1
+        out = np.zeros((n * self.lpb, self.spb), np.complex64)
+        ms, ss = self.swath("m"), self.swath("s")
+        ca, cr = np.ascontiguousarray(ca), np.ascontiguousarray(cr)
+        rc = N.load().tops_deramp_and_resample(C.byref(ms), C.byref(ss), N.ptr(ca), N.ptr(cr), b0,
+                                               bn, b0, bn, N.ptr(self.kaz, C.c_float),
+                                               N.ptr(self.kry, C.c_float), out.ctypes.data, None)
+        N.check(rc, "tops_deramp_and_resample")
+        return out
+
+    def esd_and_coh(self, ca, cr, reslave, b0=0, bn=None, esd_win=(6, 24), threshold=0.3,
+                    coh_win=(6, 24)):
+        N = native
+        bn = self.nb - 1 if bn is None else bn
+        n = bn - b0 + 1
+        res = np.zeros((n * self.lpb, self.spb), np.complex64)
+        coh = np.zeros((n * self.lpb, self.spb), np.float32)
+        shifts = np.zeros(max(n - 1, 1))
+        ms, ss = self.swath("m"), self.swath("s")
+        ca, cr = np.ascontiguousarray(ca), np.ascontiguousarray(cr)
+        reslave = np.ascontiguousarray(reslave, dtype=np.complex64)
+        rc = N.load().tops_esd_and_coh(C.byref(ms), C.byref(ss), N.ptr(ca), N.ptr(cr), b0, bn, b0,
+                                       bn, N.ptr(self.kaz, C.c_float), N.ptr(self.kry, C.c_float),
+                                       reslave.ctypes.data, esd_win[0], esd_win[1], threshold,
+                                       coh_win[0], coh_win[1], res.ctypes.data,
+                                       N.ptr(coh, C.c_float), N.ptr(shifts))
+        N.check(rc, "tops_esd_and_coh")
+        return res, coh, shifts[:n - 1]

@@ -96,3 +96,19 @@ def test_overlap_and_spectral_sep_match_oracle(tops, orc):
                             m.az_time_interval)
     b = orc.spectral_sep(kt, m.lines_per_burst, m.az_time_interval)
     assert abs(a - b) <= 1e-9 * abs(b)
+
+
+def test_read_config(tops, tmp_path):
+    """config.txt semantics of ReadInput (Func.cpp:7-130)."""
+    p = tmp_path / "config.txt"
+    p.write_text("process_dir= /data/run one\nmasterpath=/m.SAFE\nslavepath\t/s.SAFE\n"
+                 "firstsubswath=1\nlastsubswath=3\nburst0=2\nburstN=7\npolarisation=vv\n"
+                 "multi_az=4\nmulti_rg=1\nunknown=5\n")
+    N = tops.native
+    cfg = N.Config()
+    N.check(N.load().tops_read_config(str(p).encode(), C.byref(cfg)))
+    assert cfg.process_dir == b"/data/run one" and cfg.masterpath == b"/m.SAFE"
+    assert cfg.slavepath == b"/s.SAFE" and cfg.polarisation == b"vv"
+    assert (cfg.firstsubswath, cfg.lastsubswath, cfg.burst0, cfg.burstN) == (1, 3, 2, 7)
+    assert N.load().tops_read_config(b"/nonexistent/config.txt", C.byref(cfg)) == N.TOPS_ERR_ARG
+    N.load().tops_clear_error()

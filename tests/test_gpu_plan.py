@@ -1,0 +1,126 @@
+"""Batched plans (many bursts in one launch, hipGraph replay, strip-height
+tuning) against the single-burst path and the FP64 oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def batch(tops, dev):
+    """5 uneven bursts (lines 300-340, samples 700-900), each with its own
+    offsets and Doppler slope, master and slave from the generator."""
+    N = tops.native
+    L = N.load()
+    rng = np.random.default_rng(5)
+    bursts = []
+    kaz = krg = None
+    for i in range(5):
+        lines, pixels = int(rng.integers(300, 341)), int(rng.integers(700, 901))
+        az = np.array([rng.uniform(0, 0.3), 0.1, 0.05, 0, 0.05, 0])
+        rg = np.array([rng.uniform(0, 2.0), 0.4, 0.3, 0, 0.2, 0])
+        pair = tops.synth.make_pair(f"b{i}", lines, pixels, seed=100 + i, az_off=az, rg_off=rg,
+                                    dc_slope_total=float(rng.uniform(0, 60)))
+        m = pair.master
+        if kaz is None:
+            kaz, krg = N.lut_init(12, m.prf, m.azimuth_bandwidth, 2 * m.range_sampling_rate)
+        kt, tref, fdc, _ = pair.params()
+        ca, cr = pair.cpm(0)
+        bursts.append(dict(pair=pair, M=pair.burst(0, "master"), S=pair.burst(0, "slave"),
+                           kt=kt[0].copy(), tref=tref[0].copy(), fdc=fdc[0].copy(), ca=ca, cr=cr,
+                           L=lines, P=pixels, dt=m.az_time_interval))
+    ctx = L.tops_ctx_create(N.ptr(kaz, C.c_float), N.ptr(krg, C.c_float), 12)
+    keep = []
+    items = (N.BurstItem * len(bursts))()
+    for i, b in enumerate(bursts):
+        box = [0, b["P"] - 1, 0, b["L"] - 1]
+        g = N.BurstGeom((C.c_int * 4)(*box), (C.c_int * 4)(*box), 0, b["P"], b["dt"], 0.0,
+                        (C.c_double * 6)(*b["ca"]), (C.c_double * 6)(*b["cr"]))
+        d = dict(S=dev.put(b["S"]), M=dev.put(b["M"]), kt=dev.put(b["kt"]),
+                 tref=dev.put(b["tref"]), fdc=dev.put(b["fdc"]),
+                 out=dev.empty(b["L"] * b["P"] * 8), coh=dev.empty(b["L"] * b["P"] * 4))
+        keep.append(d)
+        items[i] = N.BurstItem(g, d["S"].data_ptr(), d["kt"].data_ptr(), d["tref"].data_ptr(),
+                               d["fdc"].data_ptr(), d["out"].data_ptr(), d["M"].data_ptr(),
+                               d["coh"].data_ptr())
+        b["geom"] = g
+    yield dict(bursts=bursts, ctx=ctx, items=items, dev=keep, kaz=kaz, krg=krg)
+    L.tops_ctx_destroy(ctx)
+
+
+def _single(tops, dev, ctx, b, d):
+    N = tops.native
+    L = N.load()
+    out = dev.empty(b["L"] * b["P"] * 8)
+    coh = dev.empty(b["L"] * b["P"] * 4)
+    N.check(L.tops_resample_burst(ctx, C.byref(b["geom"]), d["S"].data_ptr(), d["kt"].data_ptr(),
+                                  d["tref"].data_ptr(), d["fdc"].data_ptr(), out.data_ptr(), 0,
+                                  None))
+    N.check(L.tops_coherence(d["M"].data_ptr(), out.data_ptr(), b["P"], b["L"], 4, 1, 10, 3,
+                             coh.data_ptr(), 0, None))
+    return (dev.get(out, np.complex64, (b["L"], b["P"])),
+            dev.get(coh, np.float32, (b["L"], b["P"])))
+
+
+def test_plan_matches_single_and_oracle(tops, orc, dev, batch):
+    N = tops.native
+    L = N.load()
+    print("  plan: 5 uneven bursts", flush=True)
+    plan = L.tops_plan_create(batch["ctx"], batch["items"], 5, 3, 10, 64)
+    assert plan, N.last_error()
+    N.check(L.tops_plan_run(plan, None), "tops_plan_run")
+    for i, (b, d) in enumerate(zip(batch["bursts"], batch["dev"])):
+        got = dev.get(d["out"], np.complex64, (b["L"], b["P"]))
+        coh = dev.get(d["coh"], np.float32, (b["L"], b["P"]))
+        s_out, s_coh = _single(tops, dev, batch["ctx"], b, d)
+        # a different strip height re-anchors the deramp recurrence at other
+        # rows: the same result up to FP32 rounding
+        assert rel_err(got, s_out) <= 1e-6, i
+        assert float(np.max(np.abs(coh - s_coh))) <= 1e-6, i
+        if True:
+            box = [0, b["P"] - 1, 0, b["L"] - 1]
+            ref = orc.resample(b["S"], box, box, 0, b["dt"], 0.0, b["ca"], b["cr"], batch["kaz"],
+                               batch["krg"], b["kt"], b["tref"], b["fdc"])
+            e = rel_err(got, ref)
+            cref = orc.coherence(b["M"], got.astype(np.complex128), 4, 1, 10, 3)
+            ec = float(np.max(np.abs(coh - cref)))
+            print(f"  burst {i} {b['L']}x{b['P']}: resample {e:.2e}, coherence {ec:.2e}")
+            assert e <= 1e-5 and ec <= 1e-4
+    L.tops_plan_destroy(plan)
+
+
+def test_plan_tune_and_graph_replay(tops, dev, batch):
+    N = tops.native
+    L = N.load()
+    torch = dev.torch
+    plan = L.tops_plan_create(batch["ctx"], batch["items"], 5, 3, 10, 0)
+    assert plan, N.last_error()
+    s = torch.cuda.Stream()
+    sp = C.c_void_p(s.cuda_stream)
+    best = C.c_int(0)
+    print("  plan: tuning", flush=True)
+    N.check(L.tops_plan_tune(plan, sp, C.byref(best)), "tops_plan_tune")
+    assert best.value in (32, 64, 128, 256, 512)
+    assert L.tops_plan_strip_rows(plan) == best.value
+    N.check(L.tops_plan_run(plan, sp))
+    s.synchronize()
+    eager = [(dev.get(d["out"], np.complex64, (b["L"], b["P"])).copy(),
+              dev.get(d["coh"], np.float32, (b["L"], b["P"])).copy())
+             for b, d in zip(batch["bursts"], batch["dev"])]
+    print(f"  plan: graph capture (strip rows {best.value})", flush=True)
+    N.check(L.tops_plan_capture_graph(plan, sp), "tops_plan_capture_graph")
+    for rep in range(3):
+        for d in batch["dev"]:
+            d["out"].zero_()
+            d["coh"].zero_()
+        torch.cuda.synchronize()
+        N.check(L.tops_plan_run(plan, sp))
+        s.synchronize()
+        for (b, d), (eo, ec) in zip(zip(batch["bursts"], batch["dev"]), eager):
+            assert np.array_equal(dev.get(d["out"], np.complex64, (b["L"], b["P"])), eo), rep
+            assert np.array_equal(dev.get(d["coh"], np.float32, (b["L"], b["P"])), ec), rep
+    L.tops_plan_destroy(plan)

@@ -1,0 +1,124 @@
+"""GPU parity of the stage layer (GeometricCoreg, DerampAndResample,
+ESDAndCoh of Func.cpp:2215-2763) on a 3-burst subswath against the oracle
+functions composed the same way.
+
+Tolerances: offsets 1e-3 px, resampled samples 1e-5 of the RMS, ESD shift
+1e-4 px, coherence 1e-4.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def run(tops):
+    import tops_amd.pipeline as pl
+    pair = tops.synth.subswath_iw1(n_bursts=3, lines=400, pixels=2048, delta=0.002)
+    return pl.SubswathRun(pair)
+
+
+def _dem_window_py(run, b):
+    """RefDem::getData window of burst b with 24 posts of margin (clipped)."""
+    lat0, lon0, dlat, dlon = run.dem_geo
+    la_min, la_max, lo_min, lo_max = run.m.burst_ll[b]
+    ul0 = max(0, int(np.floor((lat0 - (la_max + 24 * dlat)) / dlat)))
+    ul1 = max(0, int(np.floor(((lo_min - 24 * dlon) - lon0) / dlon)))
+    lr0 = min(run.dem.shape[0] - 1, int(np.ceil((lat0 - (la_min - 24 * dlat)) / dlat)))
+    lr1 = min(run.dem.shape[1] - 1, int(np.ceil(((lo_max + 24 * dlon) - lon0) / dlon)))
+    win = run.dem[ul0:lr0 + 1, ul1:lr1 + 1]
+    return win, lat0 - ul0 * dlat, lon0 + ul1 * dlon, dlat, dlon
+
+
+def _surface(ca, cr):
+    gl, gp = np.meshgrid(np.linspace(-2, 2, 17), np.linspace(-2, 2, 17))
+    A = np.stack([np.ones_like(gl), gl, gp, gl * gl, gl * gp, gp * gp], axis=-1)
+    return A @ ca, A @ cr
+
+
+@pytest.fixture(scope="module")
+def stages(run):
+    print("  stages: geometric coreg", flush=True)
+    ca, cr = run.geometric_coreg()
+    print("  stages: deramp and resample", flush=True)
+    res1 = run.deramp_and_resample(ca, cr)
+    print("  stages: ESD and coherence", flush=True)
+    res2, coh, shifts = run.esd_and_coh(ca, cr, res1, coh_win=(3, 10))
+    return dict(ca=ca, cr=cr, res1=res1, res2=res2, coh=coh, shifts=shifts)
+
+
+def test_geometric_coreg_stage(orc, run, stages):
+    m, s = run.m, run.s
+    om, os_ = run.orbits["m"], run.orbits["s"]
+    for b in range(run.nb):
+        dem, lat0, lon0, dlat, dlon = _dem_window_py(run, b)
+        gm = orc.orbit_geom((om["cx"], om["cy"], om["cz"]), om["t"][5], m.az_time_interval,
+                            m.burst_first_line_time[b], m.slr_time_first_pixel, m.range_spacing,
+                            0.5 * (m.burst_first_line_time[b] + m.burst_last_line_time[b]),
+                            b * m.lines_per_burst)
+        gs = orc.orbit_geom((os_["cx"], os_["cy"], os_["cz"]), os_["t"][5], s.az_time_interval,
+                            s.burst_first_line_time[b], s.slr_time_first_pixel, s.range_spacing,
+                            0.5 * (s.burst_first_line_time[b] + s.burst_last_line_time[b]),
+                            b * s.lines_per_burst)
+        box = [0, run.spb - 1, b * run.lpb, (b + 1) * run.lpb - 1]
+        rc, oa, orr, used = orc.geocoreg(dem, lat0, lon0, dlat, dlon, -32768, gm, gs, box)
+        assert rc == 0
+        ga, gr = _surface(stages["ca"][b], stages["cr"][b])
+        ra, rr = _surface(oa, orr)
+        e = max(np.max(np.abs(ga - ra)), np.max(np.abs(gr - rr)))
+        print(f"  burst {b}: {used} posts, offset |d| = {e:.2e} px")
+        assert e <= 1e-3
+
+
+def _oracle_resample(orc, run, b, ca, cr, shift, dt, first_line):
+    kt, tref, fdc = (run.orbits["s"][k][b] for k in ("kt", "tref", "fdc"))
+    lpb, spb = run.lpb, run.spb
+    box = [0, spb - 1, b * lpb, (b + 1) * lpb - 1]
+    return orc.resample(run.s_slc[b * lpb:(b + 1) * lpb], box, box, first_line, dt, shift, ca, cr,
+                        run.kaz, run.kry, kt, tref, fdc)
+
+
+def test_deramp_and_resample_stage(orc, run, stages):
+    for b in range(run.nb):
+        ref = _oracle_resample(orc, run, b, stages["ca"][b], stages["cr"][b], 0.0,
+                               run.s.az_time_interval, b * run.lpb)
+        got = stages["res1"][b * run.lpb:(b + 1) * run.lpb]
+        e = rel_err(got, ref)
+        print(f"  burst {b}: resample max|dz|/rms = {e:.2e}")
+        assert e <= 1e-5
+
+
+def test_esd_and_coh_stage(orc, run, stages):
+    m = run.m
+    lpb, spb, nb = run.lpb, run.spb, run.nb
+    sizes, total, mx = orc.overlap_sizes(m.burst_first_line_time, m.burst_last_line_time, 0,
+                                         nb - 1, m.az_time_interval)
+    r1 = stages["res1"].astype(np.complex128)
+    M = run.m_slc
+    mf = np.concatenate([M[(i + 1) * lpb - n:(i + 1) * lpb] for i, n in enumerate(sizes)])
+    mb = np.concatenate([M[(i + 1) * lpb:(i + 1) * lpb + n] for i, n in enumerate(sizes)])
+    sf = np.concatenate([r1[(i + 1) * lpb - n:(i + 1) * lpb] for i, n in enumerate(sizes)])
+    sb = np.concatenate([r1[(i + 1) * lpb:(i + 1) * lpb + n] for i, n in enumerate(sizes)])
+    df = orc.spectral_sep(run.orbits["m"]["kt"], lpb, m.az_time_interval)
+    shifts, _ = orc.esd(mf, mb, sf, sb, sizes, spb, 6, 24, 0.3, df, m.az_time_interval)
+    shifts = np.where(np.abs(shifts) > 100, 0.0, shifts)
+    e = np.max(np.abs(stages["shifts"] - shifts))
+    print(f"  shifts gpu={stages['shifts']} oracle={shifts} |d|={e:.2e}")
+    assert e <= 1e-4
+    for b in range(nb):
+        rows = slice(b * lpb, (b + 1) * lpb)
+        if b == 0:
+            ref = r1[rows]
+        else:
+            ref = _oracle_resample(orc, run, b, stages["ca"][b], stages["cr"][b], shifts[b - 1],
+                                   m.az_time_interval, b * lpb)
+        er = rel_err(stages["res2"][rows], ref)
+        cref = orc.coherence(M[rows], stages["res2"][rows].astype(np.complex128), 4, 1, 10, 3)
+        ec = float(np.max(np.abs(stages["coh"][rows] - cref)))
+        print(f"  burst {b}: pass-2 resample {er:.2e}, coherence {ec:.2e} "
+              f"(mean coherence {stages['coh'][rows].mean():.3f})")
+        assert er <= 1e-5 and ec <= 1e-4

@@ -263,3 +263,40 @@ def test_esd_incoherent_masked(orc):
     q = lambda z: np.stack([np.round(z.real), np.round(z.imag)], -1).astype(np.int16)
     shifts, sums = orc.esd(q(rnd()), q(rnd()), rnd(), rnd(), [n], P, 6, 24, 0.9, 5400.0, DT)
     assert sums[0, 0] == 0 and sums[0, 1] == 0 and shifts[0] == 0
+
+
+def test_esd_responds_to_injected_misregistration(tops, orc):
+    """End to end on the synthetic TOPS scene (3 bursts, 100-line overlaps):
+    resample with the closed-form offsets, gather the bands, estimate.  The
+    estimate moves with the injected misregistration d by d * df_cycle/df,
+    where df_cycle = kt * (burst cycle) is the true spectral separation and
+    df = kt * lpb * dt the one GetSpectralSep uses (Func.cpp:2962-2976).  The
+    estimate at d = 0 is not 0: the reference's 12-tap LUT is not exact on
+    linear phase (first moment off by ~1e-3 line), which shows up as a
+    constant bias of a few 1e-3 lines."""
+    est = {}
+    for delta in (0.0, 0.004):
+        pair = tops.synth.subswath_iw1(n_bursts=3, lines=240, pixels=512, delta=delta)
+        m = pair.master
+        lpb, spb = m.lines_per_burst, m.samples_per_burst
+        kaz, krg = orc.lut(prf=m.prf, abw=m.azimuth_bandwidth, rsr2x=2 * m.range_sampling_rate)
+        kt, tref, fdc, _ = pair.params()
+        Ms, Rs = [], []
+        for b in range(3):
+            ca, cr = pair.cpm(b)
+            box = [0, spb - 1, b * lpb, (b + 1) * lpb - 1]
+            Rs.append(orc.resample(pair.burst(b, "slave"), box, box, b * lpb, m.az_time_interval,
+                                   0.0, ca, cr, kaz, krg, kt[b], tref[b], fdc[b]))
+            Ms.append(pair.burst(b, "master"))
+        sizes, _, _ = orc.overlap_sizes(m.burst_first_line_time, m.burst_last_line_time, 0, 2,
+                                        m.az_time_interval)
+        cat = lambda X, back: np.concatenate(
+            [X[i + 1][:n] if back else X[i][lpb - n:] for i, n in enumerate(sizes)])
+        df = orc.spectral_sep(kt, lpb, m.az_time_interval)
+        sh, _ = orc.esd(cat(Ms, 0), cat(Ms, 1), cat(Rs, 0), cat(Rs, 1), sizes, spb, 6, 24, 0.3,
+                        df, m.az_time_interval)
+        est[delta] = sh
+        ratio = (lpb - 100) / lpb
+    slope = (est[0.004] - est[0.0]) / 0.004
+    assert np.all(np.abs(slope - ratio) < 0.05), (slope, ratio)
+    assert np.all(np.abs(est[0.0]) < 0.01)
