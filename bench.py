@@ -70,6 +70,114 @@ def read_traffic(kernel, workload):
         return None
 
 
+def _surface(ca, cr):
+    gl, gp = np.meshgrid(np.linspace(-2, 2, 17), np.linspace(-2, 2, 17))
+    A = np.stack([np.ones_like(gl), gl, gp, gl * gl, gl * gp, gp * gp], axis=-1)
+    return A @ ca, A @ cr
+
+
+def bench_geocoreg(tops, torch, pair, L, N, steps, check):
+    """DEM-assisted coregistration of the burst: FP64 Newton posts/s over the
+    1-arcsec DEM window of the burst footprint (24-post margin), wall clock
+    per call (the call ends with the 34-sum readback and the 6x6 solve)."""
+    m, s = pair.master, pair.slave
+    keep = []
+
+    def geom(sw, orbit):
+        t, x, y, z = tops.synth.orbit_polyfit_inputs(orbit)
+        cx, cy, cz = N.polyfit_orbit(t, x, y, z)
+        keep.append((cx, cy, cz))
+        first, last = sw.burst_first_line_time[0], sw.burst_last_line_time[0]
+        return (cx, cy, cz), t[len(t) // 2], sw.az_time_interval, first, \
+            sw.slr_time_first_pixel, sw.range_spacing, 0.5 * (first + last)
+
+    gm, gs = geom(m, pair.morbit), geom(s, pair.sorbit)
+
+    def ng(g):
+        (cx, cy, cz), mid, dt, first, slr, rs, init = g
+        return N.OrbitGeom(N.ptr(cx), N.ptr(cy), N.ptr(cz), len(cx), mid, dt, first, slr, rs,
+                           init, 0)
+
+    dem, lat0, lon0, dlat, dlon = pair.dem_window(*m.burst_ll[0])
+    ddem = torch.from_numpy(np.ascontiguousarray(dem)).to("cuda:0")
+    box = [0, m.samples_per_burst - 1, 0, m.lines_per_burst - 1]
+    ca, cr = np.zeros(6), np.zeros(6)
+    used = C.c_int64(0)
+    ngm, ngs = ng(gm), ng(gs)
+
+    def call():
+        N.check(L.tops_geocoreg(ddem.data_ptr(), dem.shape[0], dem.shape[1], lat0, lon0, dlat,
+                                dlon, -32768, C.byref(ngm), C.byref(ngs), *box, N.ptr(ca),
+                                N.ptr(cr), C.byref(used), None, None), "geocoreg")
+
+    call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+    t = (time.perf_counter() - t0) / steps
+    out = {"posts": int(dem.size), "posts_used": int(used.value),
+           "mposts_per_s": round(dem.size / t / 1e6, 2), "ms_per_call": round(t * 1e3, 3),
+           "dem": list(dem.shape), "precision": "FP64 orbit, Newton and fit"}
+    if check:
+        from oracle import oracle as orc
+
+        def og(g):
+            (cx, cy, cz), mid, dt, first, slr, rs, init = g
+            return orc.orbit_geom((cx, cy, cz), mid, dt, first, slr, rs, init, 0)
+        t0 = time.perf_counter()
+        rc, oa, orr, _ = orc.geocoreg(dem, lat0, lon0, dlat, dlon, -32768, og(gm), og(gs), box)
+        out["oracle_s"] = round(time.perf_counter() - t0, 3)
+        ga, gr = _surface(ca, cr)
+        ra, rr = _surface(oa, orr)
+        out["offset_err_px"] = float(max(np.max(np.abs(ga - ra)), np.max(np.abs(gr - rr))))
+        out["tol_px"] = 1e-3
+    return out
+
+
+def bench_esd(torch, M, S_res, L, N, pair, kt, steps, check):
+    """Double-difference ESD over 8 overlap bands of 100 lines x the burst
+    width (the band shapes of a 9-burst subswath), band samples/s; the bands
+    are cut from the resampled burst and its master."""
+    lines, pixels = S_res.shape
+    nb, n = 8, 100
+    rows = [(i * (lines - n)) // (nb - 1) for i in range(nb)]
+    mf = np.concatenate([M[r:r + n] for r in rows])
+    mb = np.concatenate([M[r + 1:r + 1 + n] if r + 1 + n <= lines else M[r:r + n] for r in rows])
+    sf = np.concatenate([S_res[r:r + n] for r in rows])
+    sb = np.concatenate([S_res[r + 1:r + 1 + n] if r + 1 + n <= lines else S_res[r:r + n]
+                         for r in rows])
+    sizes = np.full(nb, n, np.int32)
+    d = [torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0") for a in (mf, mb, sf, sb)]
+    shifts, sums = np.zeros(nb), np.zeros(2 * nb)
+    m = pair.master
+    dt = m.az_time_interval
+    df = float(np.mean(kt)) * m.lines_per_burst * dt   # GetSpectralSep, Func.cpp:2962-2976
+
+    def call():
+        N.check(L.tops_esd_shifts(d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(),
+                                  d[3].data_ptr(), sizes.ctypes.data_as(C.POINTER(C.c_int)), nb,
+                                  pixels, 6, 24, 0.3, df, dt, N.ptr(shifts), N.ptr(sums), None),
+                "esd")
+
+    call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+    t = (time.perf_counter() - t0) / steps
+    out = {"bands": nb, "band_lines": n, "band_samples": int(nb * n * pixels),
+           "msamples_per_s": round(nb * n * pixels / t / 1e6, 2), "ms_per_call": round(t * 1e3, 3),
+           "window": [6, 24], "precision": "FP32 window sums, Kahan FP32 phasor sums"}
+    if check:
+        from oracle import oracle as orc
+        ref, _ = orc.esd(mf, mb, sf.astype(np.complex128), sb.astype(np.complex128), sizes,
+                         pixels, 6, 24, 0.3, df, dt)
+        out["shift_err_px"] = float(np.max(np.abs(shifts - ref)))
+        out["tol_px"] = 1e-4
+    return out
+
+
 def run_iw_pair(args, tops, torch):
     """27 uneven bursts (3 subswaths x 9; 21000 / 25000 / 24500 samples,
     1480-1520 lines each) in one batched plan: one resample launch and one
@@ -175,6 +283,9 @@ def main():
     ap.add_argument("--workload", default="single_burst_iw1")
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (twice) and exit: "
+                         "the reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -182,6 +293,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "launches": 2}))
+        return
 
     tops = load_package()
     if args.workload == "iw_pair":
@@ -314,6 +435,9 @@ def main():
                                   "cpu": cpu_model()}
         result["error_vs_oracle"] = {"resample_max_abs_over_rms": err, "tol_resample": 1e-5,
                                      "coherence_max_abs": cerr, "tol_coherence": 1e-4}
+    print("[bench] geocoreg and ESD", file=sys.stderr, flush=True)
+    result["geocoreg"] = bench_geocoreg(tops, torch, pair, L, N, 5, not args.no_cpu_baseline)
+    result["esd"] = bench_esd(torch, M, got, L, N, pair, kt[0], 10, not args.no_cpu_baseline)
     L.tops_ctx_destroy(ctx)
     print(json.dumps(result))
 

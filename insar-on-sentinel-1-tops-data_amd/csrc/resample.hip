@@ -18,10 +18,14 @@
 //   resample_fp64_*  plain all-FP64 per-stage form (deramp kernel writing an
 //                    FP64 deramped burst + phase to HBM, then one thread per
 //                    output reading 144+144 taps from HBM): the baseline.
-//   resample_fast    fused: a 16x64 output tile stages its deramped slave
-//                    window in LDS (FP64 phase reduced mod 2*pi, FP32
-//                    phasor), FP32 sinc MACs, and the reramp phase from the
-//                    FP64 moment identity below instead of 144 FP64 taps.
+//   resample_fast    fused column walker: each 64-lane wave walks a strip
+//                    of output rows; every lane deramps one slave column on
+//                    the fly (FP64 phasor recurrence, FP32 samples), the
+//                    wave shares one 12-tap azimuth pass per distinct
+//                    azimuth key through LDS, and each output does its 12
+//                    range taps over those sums (FP32 MACs) and over the
+//                    azimuth-interpolated phase from the moment identity
+//                    below (FP64), then the reramp.
 //
 // Moment identity: phi is quadratic in the burst-relative line y for a fixed
 // column, phi(y, x) = a(x) + b(x) y + c(x) y^2 with
@@ -204,10 +208,14 @@ __device__ inline void load_taps(const float* lut, int row, float k[kTaps])
     k[8] = c.x; k[9] = c.y; k[10] = c.z; k[11] = c.w;
 }
 
+// out = (sr + j si) * exp(-j phi).  The phase goes to revolutions and is
+// reduced to [-1/2, 1/2] in FP64; the hardware sine and cosine take
+// revolutions directly (absolute error about 1.3e-7 on that interval).
 __device__ inline tops_cf32 reramp(float sr, float si, double phi)
 {
-    float sn, cs;
-    sincosf(reduce_2pi(phi), &sn, &cs);
+    double t = phi * kInvTwoPi;
+    float tf = (float)(t - rint(t));
+    float sn = __builtin_amdgcn_sinf(tf), cs = __builtin_amdgcn_cosf(tf);
     tops_cf32 o;
     o.x = sr * cs + si * sn;
     o.y = si * cs - sr * sn;
@@ -254,7 +262,8 @@ __device__ __noinline__ tops_cf32 interior_direct(const ResampleParams& p, int i
     return reramp(sr, si, phi);
 }
 
-// Column state of one lane: deramp recurrence along its slave column.
+// Column state of one lane: deramp recurrence along its slave column, with
+// the raw samples of the next four lines already requested from HBM.
 struct ColumnWalker {
     int xcol;            // absolute slave column
     bool in_box;         // column inside the slave box
@@ -262,7 +271,17 @@ struct ColumnWalker {
     double pr, pi;       // exp(j phi(ynext))
     double ir, ii;       // exp(j (b + c (2 ynext + 1)))
     double qr, qi;       // exp(j 2c)
-    int ynext;           // absolute slave line produced next
+    int ynext;           // absolute slave line deramped next
+    int pend[4];         // raw CInt16 of lines ynext .. ynext+3
+
+    // raw CInt16 (packed re | im << 16) of line y of this column, 0 outside
+    // the box (a zero sample deramps to zero)
+    __device__ int fetch(const ResampleParams& p, int y) const
+    {
+        if (!in_box || y < p.sb[2] || y > p.sb[3]) return 0;
+        return reinterpret_cast<const int*>(p.slave)[(size_t)(y - p.sb[2]) * p.s_pixels +
+                                                     (xcol - p.sb[0])];
+    }
 
     __device__ void anchor(const ResampleParams& p, int y_abs)
     {
@@ -277,23 +296,25 @@ struct ColumnWalker {
         ir = cc; ii = s;
     }
 
-    // deramped sample of row ynext, then advance the recurrence
-    __device__ float2 next(const ResampleParams& p)
+    // deramped sample of line ynext from its raw value, then advance
+    __device__ float2 deramp(int raw)
     {
-        float2 d = make_float2(0.f, 0.f);
-        int y = ynext;
-        if (in_box && y >= p.sb[2] && y <= p.sb[3]) {
-            tops_ci16 s = p.slave[(size_t)(y - p.sb[2]) * p.s_pixels + (xcol - p.sb[0])];
-            float cr = (float)pr, ci = (float)pi;
-            float re = (float)s.re, im = (float)s.im;
-            d = make_float2(re * cr - im * ci, re * ci + im * cr);
-        }
+        const float re = (float)(short)(raw & 0xffff), im = (float)(short)(raw >> 16);
+        const float cr = (float)pr, ci = (float)pi;
+        float2 d = make_float2(re * cr - im * ci, re * ci + im * cr);
         double nr = pr * ir - pi * ii, ni = pr * ii + pi * ir;
         pr = nr; pi = ni;
         double jr = ir * qr - ii * qi, ji = ir * qi + ii * qr;
         ir = jr; ii = ji;
-        ynext = y + 1;
+        ynext++;
         return d;
+    }
+
+    // request the four lines after the ones deramped so far
+    __device__ void prefetch(const ResampleParams& p)
+    {
+#pragma unroll
+        for (int k = 0; k < 4; k++) pend[k] = fetch(p, ynext + k);
     }
 };
 
@@ -396,15 +417,20 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
             if (o < 0 || o > 64) {
                 wbase = iy_min - kHalf;
                 w.anchor(p, wbase);
+                int raw[WK_WIN];
 #pragma unroll
-                for (int k = 0; k < WK_WIN; k++) win[k] = w.next(p);
+                for (int k = 0; k < WK_WIN; k++) raw[k] = w.fetch(p, wbase + k);
+#pragma unroll
+                for (int k = 0; k < WK_WIN; k++) win[k] = w.deramp(raw[k]);
+                w.prefetch(p);
                 o = 0;
             }
             while (o > 3) {   // slide by 4 rows
 #pragma unroll
                 for (int k = 0; k < WK_WIN - 4; k++) win[k] = win[k + 4];
 #pragma unroll
-                for (int k = WK_WIN - 4; k < WK_WIN; k++) win[k] = w.next(p);
+                for (int k = 0; k < 4; k++) win[WK_WIN - 4 + k] = w.deramp(w.pend[k]);
+                w.prefetch(p);
                 wbase += 4;
                 o -= 4;
             }
@@ -510,7 +536,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     }
 }
 
-__global__ void __launch_bounds__(64) resample_fast_kernel(ResampleParams p, int strip_rows)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) resample_fast_kernel(ResampleParams p, int strip_rows)
 {
     __shared__ WalkLds sm;
     resample_strip(p, blockIdx.x, blockIdx.y, strip_rows, sm);
@@ -518,7 +544,7 @@ __global__ void __launch_bounds__(64) resample_fast_kernel(ResampleParams p, int
 
 // Batched form: one workgroup per entry of a strip table covering many
 // bursts (work[i] = {burst, strip x, strip y, strip rows}).
-__global__ void __launch_bounds__(64) resample_batch_kernel(const ResampleParams* items,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) resample_batch_kernel(const ResampleParams* items,
                                                             const int4* work)
 {
     __shared__ WalkLds sm;

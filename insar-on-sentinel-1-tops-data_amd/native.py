@@ -99,6 +99,15 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+    # The torch wheel carries its own HIP runtime.  Loaded first, it is the
+    # runtime this library binds to (same soname), so device pointers and
+    # streams from torch are valid here.  Loaded after this library, it would
+    # be a second runtime instance in the process, and whichever initialises
+    # second finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.tops_last_error.restype = C.c_int
     L.tops_last_error_message.restype = C.c_char_p

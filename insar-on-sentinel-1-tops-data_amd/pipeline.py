@@ -38,7 +38,7 @@ class SubswathRun:
             self.orbits[key] = dict(t=np.ascontiguousarray(t), cx=cx, cy=cy, cz=cz, kt=kt,
                                     tref=tref, fdc=fdc, speed=speed)
         # InKernelInit with the master's PRF / ABW / RSR (Main.cpp:75-76)
-        self.kaz, self.kry = N.lut_init(12, m.prf, m.azimuth_bandwidth,
+        self.kaz, self.krg = N.lut_init(12, m.prf, m.azimuth_bandwidth,
                                         2 * m.range_sampling_rate)
         # DEM covering every burst
         ll = m.burst_ll
@@ -98,7 +98,7 @@ class SubswathRun:
         ca, cr = np.ascontiguousarray(ca), np.ascontiguousarray(cr)
         rc = N.load().tops_deramp_and_resample(C.byref(ms), C.byref(ss), N.ptr(ca), N.ptr(cr), b0,
                                                bn, b0, bn, N.ptr(self.kaz, C.c_float),
-                                               N.ptr(self.kry, C.c_float), out.ctypes.data, None)
+                                               N.ptr(self.krg, C.c_float), out.ctypes.data, None)
         N.check(rc, "tops_deramp_and_resample")
         return out
 
@@ -114,7 +114,7 @@ class SubswathRun:
         ca, cr = np.ascontiguousarray(ca), np.ascontiguousarray(cr)
         reslave = np.ascontiguousarray(reslave, dtype=np.complex64)
         rc = N.load().tops_esd_and_coh(C.byref(ms), C.byref(ss), N.ptr(ca), N.ptr(cr), b0, bn, b0,
-                                       bn, N.ptr(self.kaz, C.c_float), N.ptr(self.kry, C.c_float),
+                                       bn, N.ptr(self.kaz, C.c_float), N.ptr(self.krg, C.c_float),
                                        reslave.ctypes.data, esd_win[0], esd_win[1], threshold,
                                        coh_win[0], coh_win[1], res.ctypes.data,
                                        N.ptr(coh, C.c_float), N.ptr(shifts))

@@ -28,6 +28,17 @@ def test_library_exports_header(tops):
     assert set(tops.native.EXPORTS) <= set(names)
 
 
+def test_single_hip_runtime(tops):
+    """The library shares torch's HIP runtime: one libamdhip64 and one HSA
+    runtime mapped in the process (a second instance finds no device)."""
+    tops.native.load()
+    with open("/proc/self/maps") as f:
+        paths = {ln.split()[-1] for ln in f if "libamdhip64" in ln or "libhsa-runtime64" in ln}
+    hip = [p for p in paths if "libamdhip64" in p]
+    hsa = [p for p in paths if "libhsa-runtime64" in p]
+    assert len(hip) == 1 and len(hsa) == 1, sorted(paths)
+
+
 def test_status_api(tops):
     L = tops.native.load()
     L.tops_clear_error()

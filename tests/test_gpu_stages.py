@@ -79,7 +79,7 @@ def _oracle_resample(orc, run, b, ca, cr, shift, dt, first_line):
     lpb, spb = run.lpb, run.spb
     box = [0, spb - 1, b * lpb, (b + 1) * lpb - 1]
     return orc.resample(run.s_slc[b * lpb:(b + 1) * lpb], box, box, first_line, dt, shift, ca, cr,
-                        run.kaz, run.kry, kt, tref, fdc)
+                        run.kaz, run.krg, kt, tref, fdc)
 
 
 def test_deramp_and_resample_stage(orc, run, stages):
@@ -121,4 +121,11 @@ def test_esd_and_coh_stage(orc, run, stages):
         ec = float(np.max(np.abs(stages["coh"][rows] - cref)))
         print(f"  burst {b}: pass-2 resample {er:.2e}, coherence {ec:.2e} "
               f"(mean coherence {stages['coh'][rows].mean():.3f})")
+        if er > 1e-5:
+            got = stages["res2"][rows]
+            d = np.abs(got - ref) / np.sqrt(np.mean(np.abs(ref) ** 2))
+            bad = np.argwhere(d > 1e-5)
+            print(f"    {len(bad)} samples off; rows {np.unique(bad[:, 0])[:12].tolist()} "
+                  f"cols {np.unique(bad[:, 1])[:12].tolist()}; first {bad[0].tolist()}: "
+                  f"got {got[tuple(bad[0])]} ref {ref[tuple(bad[0])]}")
         assert er <= 1e-5 and ec <= 1e-4
