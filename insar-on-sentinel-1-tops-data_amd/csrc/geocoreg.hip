@@ -147,14 +147,22 @@ __global__ void __launch_bounds__(GEO_THREADS) geocoreg_kernel(
     }
 }
 
-__global__ void geocoreg_final_kernel(double* partials, int nblocks)
+// One workgroup per sum.  Thread t adds partials t, t+256, ... in order, then
+// a fixed LDS tree adds the 256 thread sums: the result depends only on
+// nblocks, never on timing.
+__global__ void __launch_bounds__(256) geocoreg_final_kernel(double* partials, int nblocks)
 {
-    // one thread per sum, fixed order: deterministic
-    int k = threadIdx.x;
-    if (k >= kGeoSums) return;
+    __shared__ double sh[256];
+    const int k = blockIdx.x, t = threadIdx.x;
     double v = 0.0;
-    for (int b = 0; b < nblocks; b++) v += partials[(size_t)b * kGeoSums + k];
-    partials[(size_t)nblocks * kGeoSums + k] = v;
+    for (int b = t; b < nblocks; b += 256) v += partials[(size_t)b * kGeoSums + k];
+    sh[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) partials[(size_t)nblocks * kGeoSums + k] = sh[0];
 }
 
 int geocoreg_blocks(long posts)
@@ -174,7 +182,8 @@ int launch_geocoreg(const int32_t* dem, int lines, int pixels, double lat0, doub
                        pixels, lat0, lon0, dlat, dlon, nodata, m, s, xmin, xmax, ymin, ymax,
                        d_partials, d_lp);
     TOPS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(geocoreg_final_kernel, dim3(1), dim3(64), 0, st, d_partials, nblocks);
+    hipLaunchKernelGGL(geocoreg_final_kernel, dim3(kGeoSums), dim3(256), 0, st, d_partials,
+                       nblocks);
     TOPS_HIP(hipGetLastError());
     return TOPS_OK;
 }

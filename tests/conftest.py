@@ -69,3 +69,52 @@ def rel_err(got, ref):
     """max |got - ref| / RMS |ref| (the resample tolerance metric)"""
     rms = np.sqrt(np.mean(np.abs(ref) ** 2))
     return float(np.max(np.abs(got - ref)) / rms)
+
+
+def dem_window_py(run, b):
+    """RefDem::getData window (DataTypes.cpp:267-300) of master burst b with
+    24 posts of margin, clipped to the DEM: (dem, lat0, lon0, dlat, dlon)."""
+    lat0, lon0, dlat, dlon = run.dem_geo
+    la_min, la_max, lo_min, lo_max = run.m.burst_ll[b]
+    ul0 = max(0, int(np.floor((lat0 - (la_max + 24 * dlat)) / dlat)))
+    ul1 = max(0, int(np.floor(((lo_min - 24 * dlon) - lon0) / dlon)))
+    lr0 = min(run.dem.shape[0] - 1, int(np.ceil((lat0 - (la_min - 24 * dlat)) / dlat)))
+    lr1 = min(run.dem.shape[1] - 1, int(np.ceil(((lo_max + 24 * dlon) - lon0) / dlon)))
+    win = run.dem[ul0:lr0 + 1, ul1:lr1 + 1]
+    return win, lat0 - ul0 * dlat, lon0 + ul1 * dlon, dlat, dlon
+
+
+def offset_surface(ca, cr):
+    """Fitted slave line / pixel over a 17x17 grid of normalised master
+    coordinates in [-2, 2] (the burst box maps to [-2, 2])."""
+    gl, gp = np.meshgrid(np.linspace(-2, 2, 17), np.linspace(-2, 2, 17))
+    A = np.stack([np.ones_like(gl), gl, gp, gl * gl, gl * gp, gp * gp], axis=-1)
+    return A @ ca, A @ cr
+
+
+def oracle_geocoreg_burst(orc, run, b):
+    """Oracle coregistration polynomials of burst b, set up as
+    GeometricCoreg (Func.cpp:2215-2297) does."""
+    m, s = run.m, run.s
+    om, os_ = run.orbits["m"], run.orbits["s"]
+    dem, lat0, lon0, dlat, dlon = dem_window_py(run, b)
+    gm = orc.orbit_geom((om["cx"], om["cy"], om["cz"]), om["t"][5], m.az_time_interval,
+                        m.burst_first_line_time[b], m.slr_time_first_pixel, m.range_spacing,
+                        0.5 * (m.burst_first_line_time[b] + m.burst_last_line_time[b]),
+                        b * m.lines_per_burst)
+    gs = orc.orbit_geom((os_["cx"], os_["cy"], os_["cz"]), os_["t"][5], s.az_time_interval,
+                        s.burst_first_line_time[b], s.slr_time_first_pixel, s.range_spacing,
+                        0.5 * (s.burst_first_line_time[b] + s.burst_last_line_time[b]),
+                        b * s.lines_per_burst)
+    box = [0, run.spb - 1, b * run.lpb, (b + 1) * run.lpb - 1]
+    return orc.geocoreg(dem, lat0, lon0, dlat, dlon, -32768, gm, gs, box)
+
+
+def oracle_resample_burst(orc, run, b, ca, cr, shift, dt, row0=0, row1=None):
+    """Oracle resample of slave burst b onto master burst b (rows row0..row1
+    of the burst), as DerampAndResample / ESDAndCoh set it up."""
+    kt, tref, fdc = (run.orbits["s"][k][b] for k in ("kt", "tref", "fdc"))
+    lpb, spb = run.lpb, run.spb
+    box = [0, spb - 1, b * lpb, (b + 1) * lpb - 1]
+    return orc.resample(run.s_slc[b * lpb:(b + 1) * lpb], box, box, b * lpb, dt, shift, ca, cr,
+                        run.kaz, run.krg, kt, tref, fdc, row0, row1)
