@@ -193,12 +193,6 @@ constexpr int WK_ROWS = 128;   // output rows per strip
 constexpr int WK_WIN = 16;     // register window rows
 constexpr int WK_KEYS = 4;     // azimuth keys per row served by the shared pass
 
-__device__ inline int wave_min(int v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
 __device__ inline void load_taps(const float* lut, int row, float k[kTaps])
 {
     const float4* q = reinterpret_cast<const float4*>(lut + row * kTaps);
@@ -223,13 +217,19 @@ __device__ inline tops_cf32 reramp(float sr, float si, double phi)
 }
 
 // Interior output computed straight from HBM (outputs the shared pass does
-// not serve).
-__device__ __noinline__ tops_cf32 interior_direct(const ResampleParams& p, int iy, int fy, int ix,
-                                                 int fx)
+// not serve).  Out of line to keep the walker's register budget; it takes
+// plain values, not the parameter block, so the kernel never needs the
+// block's address (which would put a copy of it in scratch memory).
+__device__ __noinline__ tops_cf32 interior_direct(const tops_ci16* slave, const double* kt,
+                                                 const double* tref, const double* fdc,
+                                                 const double* mom, const float* kaz,
+                                                 const float* krg, int s_pixels, int sx0,
+                                                 int sy0, int first_line, double dt, int iy,
+                                                 int fy, int ix, int fx)
 {
     float kl[kTaps], kp[kTaps];
-    load_taps(p.kaz, fy, kl);
-    load_taps(p.krg, fx, kp);
+    load_taps(kaz, fy, kl);
+    load_taps(krg, fx, kp);
     float sr = 0.f, si = 0.f;
     double A = 0.0, B = 0.0, C = 0.0;
     for (int j = 0; j < kTaps; j++) {
@@ -237,14 +237,16 @@ __device__ __noinline__ tops_cf32 interior_direct(const ResampleParams& p, int i
         float tr = 0.f, ti = 0.f;
         for (int i = 0; i < kTaps; i++) {
             int xa = ix - kHalf + i;
-            double a, b, c;
-            phase_coeffs(p, xa, a, b, c);
-            double y = (double)(ya - p.first_line);
+            double k = kt[xa], t = tref[xa], f = fdc[xa];
+            double a = -kPi * k * (t * t);
+            double b = 2.0 * kPi * dt * (k * t - f);
+            double c = -kPi * k * (dt * dt);
+            double y = (double)(ya - first_line);
             float sn, cs;
             sincosf(reduce_2pi(fma(y, fma(y, c, b), a)), &sn, &cs);
-            tops_ci16 s = p.slave[(size_t)(ya - p.sb[2]) * p.s_pixels + (xa - p.sb[0])];
-            float dr = (float)s.re * cs - (float)s.im * sn;
-            float di = (float)s.re * sn + (float)s.im * cs;
+            tops_ci16 v = slave[(size_t)(ya - sy0) * s_pixels + (xa - sx0)];
+            float dr = (float)v.re * cs - (float)v.im * sn;
+            float di = (float)v.re * sn + (float)v.im * cs;
             tr = fmaf(kp[i], dr, tr);
             ti = fmaf(kp[i], di, ti);
             if (j == 0) {
@@ -256,8 +258,8 @@ __device__ __noinline__ tops_cf32 interior_direct(const ResampleParams& p, int i
         sr = fmaf(kl[j], tr, sr);
         si = fmaf(kl[j], ti, si);
     }
-    double4 m = reinterpret_cast<const double4*>(p.mom)[fy];
-    double y0 = (double)(iy - p.first_line);
+    double4 m = reinterpret_cast<const double4*>(mom)[fy];
+    double y0 = (double)(iy - first_line);
     double phi = m.x * A + fma(y0, m.x, m.y) * B + (y0 * y0 * m.x + 2.0 * y0 * m.y + m.z) * C;
     return reramp(sr, si, phi);
 }
@@ -406,12 +408,19 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
         const bool servable = reg == 2 && base >= 0 && base <= 64 - kTaps;
         const int key = servable ? iy * kLutRows + fy : INT_MAX;
 
-        // ---- distinct azimuth keys of the row, smallest first
-        int kcur = __builtin_amdgcn_readfirstlane(wave_min(key));
+        // ---- distinct azimuth keys of the row.  Keys change slowly along
+        // the row, so a key is listed at each lane where it differs from the
+        // lane to its left: one neighbour exchange and a ballot, then a
+        // scalar walk over the set bits (a key that reappears further right
+        // is recognised and not computed twice).
+        const int kleft = __shfl_up(key, 1, 64);
+        const unsigned long long heads = __ballot(key != INT_MAX && (lane == 0 || kleft != key));
         int mybuf = -1;
-        const bool has_window = kcur != INT_MAX;
+        const bool has_window = heads != 0ull;
         if (has_window) {
-            const int iy_min = kcur / kLutRows;
+            int iy_min = INT_MAX;
+            for (unsigned long long h = heads; h; h &= h - 1)
+                iy_min = min(iy_min, __builtin_amdgcn_readlane(key, __builtin_ctzll(h)) / kLutRows);
             // window: rows iy_min-5 .. iy_min+7 must be held
             int o = wbase == INT_MIN ? -1 : iy_min - kHalf - wbase;
             if (o < 0 || o > 64) {
@@ -434,31 +443,36 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
                 wbase += 4;
                 o -= 4;
             }
+            static_assert(WK_KEYS == 4, "served-key registers below");
+            int sv0 = INT_MAX, sv1 = INT_MAX, sv2 = INT_MAX;   // keys served so far
             int nk = 0;
-            while (kcur != INT_MAX && nk < WK_KEYS) {
-                const int iyq = kcur / kLutRows, fyq = kcur - iyq * kLutRows;
+            for (unsigned long long h = heads; h && nk < WK_KEYS; h &= h - 1) {
+                const int kq = __builtin_amdgcn_readlane(key, __builtin_ctzll(h));
+                const bool dup = kq == sv0 || kq == sv1 || kq == sv2;
+                const int iyq = kq / kLutRows, fyq = kq - iyq * kLutRows;
                 const int oq = o + (iyq - iy_min);
-                if (oq <= 4) {
-                    float kl[kTaps];
-                    load_taps(p.kaz, fyq, kl);
-                    float2 u;
-                    switch (oq) {
-                        case 0: u = az_dot<0>(win, kl); break;
-                        case 1: u = az_dot<1>(win, kl); break;
-                        case 2: u = az_dot<2>(win, kl); break;
-                        case 3: u = az_dot<3>(win, kl); break;
-                        default: u = az_dot<4>(win, kl); break;
-                    }
-                    double4 m = reinterpret_cast<const double4*>(p.mom)[fyq];
-                    double y0 = (double)(iyq - p.first_line);
-                    double bq = fma(y0, m.x, m.y);
-                    double cq = y0 * y0 * m.x + 2.0 * y0 * m.y + m.z;
-                    sm.u[nk][lane] = u;
-                    sm.pu[nk][lane] = fma(m.x, w.a, fma(bq, w.b, cq * w.c));
-                    if (key == kcur) mybuf = nk;
-                    nk++;
+                if (dup || oq > 4) continue;
+                float kl[kTaps];
+                load_taps(p.kaz, fyq, kl);
+                float2 u;
+                switch (oq) {
+                    case 0: u = az_dot<0>(win, kl); break;
+                    case 1: u = az_dot<1>(win, kl); break;
+                    case 2: u = az_dot<2>(win, kl); break;
+                    case 3: u = az_dot<3>(win, kl); break;
+                    default: u = az_dot<4>(win, kl); break;
                 }
-                kcur = __builtin_amdgcn_readfirstlane(wave_min(key > kcur ? key : INT_MAX));
+                double4 m = reinterpret_cast<const double4*>(p.mom)[fyq];
+                double y0 = (double)(iyq - p.first_line);
+                double bq = fma(y0, m.x, m.y);
+                double cq = y0 * y0 * m.x + 2.0 * y0 * m.y + m.z;
+                sm.u[nk][lane] = u;
+                sm.pu[nk][lane] = fma(m.x, w.a, fma(bq, w.b, cq * w.c));
+                if (key == kq) mybuf = nk;
+                if (nk == 0) sv0 = kq;
+                else if (nk == 1) sv1 = kq;
+                else if (nk == 2) sv2 = kq;
+                nk++;
             }
         }
         __syncthreads();
@@ -528,7 +542,9 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
                 }
                 o = reramp(sr, si, ph);
             } else if (reg == 2) {
-                o = interior_direct(p, iy, fy, ix, fx);
+                o = interior_direct(p.slave, p.kt, p.tref, p.fdc, p.mom, p.kaz, p.krg,
+                                    p.s_pixels, p.sb[0], p.sb[2], p.first_line, p.dt, iy, fy,
+                                    ix, fx);
             }
             p.out[(size_t)r * p.m_pixels + c] = o;
         }
