@@ -13,6 +13,7 @@
 // FP64 form: one thread per output, direct FP64 window sums from HBM.
 #include "tops_internal.hpp"
 #include "tile_window.hpp"
+#include "coherence_walk.hpp"
 
 namespace tops {
 
@@ -88,6 +89,55 @@ __global__ void __launch_bounds__(256) coherence_batch_kernel(const CohItem* ite
     coherence_tile(it.m, it.s, it.w, it.h, dx, dy, win_rg, win_az, it.coh, wk.y, wk.z, smem);
 }
 
+// Walking form for the 3 az x 10 rg window centred as the stage layer
+// centres it (dx = 4, dy = 1): 256 threads = 4 waves; a wave covers
+// 256 columns x CW_ROWS rows, the workgroup 256 columns x 4 CW_ROWS rows.
+using CohWalk3x10 = CohWalk<10, 3, 4, 1>;
+
+__device__ __forceinline__ void coherence_walk_block(const tops_ci16* M, const tops_cf32* S,
+                                                     int w, int h, float* coh, int bx, int by,
+                                                     int vec)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c0 = CW_COLS * (bx * 64 + lane);
+    const int r0 = (by * 4 + wv) * CW_ROWS;
+    if (c0 >= w || r0 >= h) return;
+    CohWalk3x10::walk(M, S, w, h, coh, c0, r0, min(r0 + CW_ROWS, h), vec != 0);
+}
+
+__global__ void __launch_bounds__(256) coherence_walk_kernel(const tops_ci16* M, const tops_cf32* S,
+                                                             int w, int h, float* coh, int vec)
+{
+    coherence_walk_block(M, S, w, h, coh, blockIdx.x, blockIdx.y, vec);
+}
+
+// Batched form: work[i] = {item, block x, block y, vec}.
+__global__ void __launch_bounds__(256) coherence_walk_batch_kernel(const CohItem* items,
+                                                                   const int4* work)
+{
+    const int4 wk = work[blockIdx.x];
+    const CohItem& it = items[wk.x];
+    coherence_walk_block(it.m, it.s, it.w, it.h, it.coh, wk.y, wk.z, wk.w);
+}
+
+bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az)
+{
+    return win_rg == 10 && win_az == 3 && dx == 4 && dy == 1;
+}
+
+int coherence_walk_vec(const void* m, const void* s, const void* coh, int w)
+{
+    return (w % 4 == 0) && ((uintptr_t)m % 16 == 0) && ((uintptr_t)s % 16 == 0) &&
+           ((uintptr_t)coh % 16 == 0);
+}
+
+int coherence_walk_count(int w, int h, int* nx, int* ny)
+{
+    *nx = (w + 64 * CW_COLS - 1) / (64 * CW_COLS);
+    *ny = (h + 4 * CW_ROWS - 1) / (4 * CW_ROWS);
+    return *nx * *ny;
+}
+
 int coherence_tile_count(int w, int h, int* nx, int* ny)
 {
     *nx = (w + TW_COLS - 1) / TW_COLS;
@@ -99,6 +149,12 @@ int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_wor
                            int win_rg, int win_az, hipStream_t st)
 {
     if (n_work <= 0) return TOPS_OK;
+    if (coherence_walk_fits(dx, dy, win_rg, win_az)) {
+        hipLaunchKernelGGL(coherence_walk_batch_kernel, dim3(n_work), dim3(256), 0, st, d_items,
+                           d_work);
+        TOPS_HIP(hipGetLastError());
+        return TOPS_OK;
+    }
     if (!tile_window_fits(win_rg, win_az))
         return set_error(TOPS_ERR_ARG, "batched coherence: window %dx%d exceeds the LDS tile",
                          win_az, win_rg);
@@ -116,6 +172,11 @@ int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h, int d
         dim3 g((w + 255) / 256, h);
         hipLaunchKernelGGL(coherence_fp64_kernel, g, dim3(256), 0, st, m, s, w, h, dx, dy,
                            win_rg, win_az, coh);
+    } else if (coherence_walk_fits(dx, dy, win_rg, win_az)) {
+        int nx, ny;
+        coherence_walk_count(w, h, &nx, &ny);
+        hipLaunchKernelGGL(coherence_walk_kernel, dim3(nx, ny), dim3(256), 0, st, m, s, w, h, coh,
+                           coherence_walk_vec(m, s, coh, w));
     } else {
         dim3 g((w + TW_COLS - 1) / TW_COLS, (h + TW_ROWS - 1) / TW_ROWS);
         hipLaunchKernelGGL(coherence_fast_kernel, g, dim3(256),

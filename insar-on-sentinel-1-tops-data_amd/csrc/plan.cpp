@@ -74,10 +74,16 @@ int fill_params(tops_plan* p)
         c.coh = it.coh;
         c.w = prm[i].m_pixels;
         c.h = prm[i].m_lines;
-        int nx, ny;
-        coherence_tile_count(c.w, c.h, &nx, &ny);
+        int nx, ny, tag = 0;
+        if (coherence_walk_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2, p->coh_win_rg,
+                                p->coh_win_az)) {
+            coherence_walk_count(c.w, c.h, &nx, &ny);
+            tag = coherence_walk_vec(c.m, c.s, c.coh, c.w);
+        } else {
+            coherence_tile_count(c.w, c.h, &nx, &ny);
+        }
         for (int y = 0; y < ny; y++)
-            for (int x = 0; x < nx; x++) wc.push_back(make_int4((int)ci.size(), x, y, 0));
+            for (int x = 0; x < nx; x++) wc.push_back(make_int4((int)ci.size(), x, y, tag));
         ci.push_back(c);
     }
     p->n_work_coh = (int)wc.size();

@@ -109,6 +109,11 @@ struct CohItem {
 int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_work, int dx,
                            int dy, int win_rg, int win_az, hipStream_t st);
 int coherence_tile_count(int w, int h, int* nx, int* ny);
+// walking form (3 az x 10 rg window, dx = 4, dy = 1); batched work items are
+// {item, block x, block y, coherence_walk_vec(...)}
+bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az);
+int coherence_walk_vec(const void* m, const void* s, const void* coh, int w);
+int coherence_walk_count(int w, int h, int* nx, int* ny);
 
 // launchers (esd.hip): writes 2*n_overlap double sums to d_sums
 int launch_esd(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* sf,

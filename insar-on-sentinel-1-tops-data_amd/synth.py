@@ -246,9 +246,10 @@ class Pair:
         return cpm_from_field(self.az_off, self.rg_off, L, P, b * self.slave.lines_per_burst)
 
     def burst(self, b, which):
-        """CInt16 burst b of 'master' or 'slave' as int16 [L, P, 2]."""
+        """CInt16 burst b of 'master' or 'slave' as int16 [L, P, 2], ramped
+        with that swath's own Doppler / FM parameters."""
         S = native.load_synth()
-        sw = self.master
+        sw = self.slave if which == "slave" else self.master
         L, P = sw.lines_per_burst, sw.samples_per_burst
         kt, tref, fdc, _ = deramp_params(sw, self.speed)
         kt_b = np.ascontiguousarray(kt[b])
@@ -296,7 +297,7 @@ class Pair:
 
 
 def _swath(name_seed, lines, pixels, n_bursts, t_first, cycle_lines, slr0, dc_slope_total,
-           orbit, scene_center):
+           orbit, scene_center, dc0=20.0):
     dt = AZ_TIME_INTERVAL
     first = t_first + np.arange(n_bursts) * cycle_lines * dt
     # a hair less than the cycle so EstOverlapSize's floor is not marginal
@@ -305,7 +306,7 @@ def _swath(name_seed, lines, pixels, n_bursts, t_first, cycle_lines, slr0, dc_sl
     two_way0 = 2 * slr0
     drange = 2 * RANGE_SPACING / SOL
     c1 = dc_slope_total / (pixels - 1) / drange
-    dc = np.tile([two_way0, 20.0, c1, 0.0], (n_bursts, 1))
+    dc = np.tile([two_way0, dc0, c1, 0.0], (n_bursts, 1))
     fm = np.tile([two_way0, -2300.0, 0.0, 0.0], (n_bursts, 1))
     ll = np.zeros((n_bursts, 4))
     for b in range(n_bursts):
@@ -329,9 +330,11 @@ def _swath(name_seed, lines, pixels, n_bursts, t_first, cycle_lines, slr0, dc_sl
 def make_pair(name, lines, pixels, n_bursts=1, overlap=100, seed=20260101, stream=0,
               az_off=BILINEAR_AZ, rg_off=BILINEAR_RG, delta=0.0, phase_noise=0.2,
               dem_spacing=3.0, dc_slope_total=5e-3 * 2047, scene_center=(30.0, 100.0),
-              slr_shift=0.0):
+              slr_shift=0.0, slave_dc0=20.0):
     """Burst-aligned master/slave pair; the slave orbit carries a ~120 m
-    baseline and a small along-track delay so geocoreg sees real offsets."""
+    baseline and a small along-track delay so geocoreg sees real offsets.
+    slave_dc0: the slave's Doppler centroid at the first range sample (Hz);
+    the master's is 20 Hz."""
     tc = 43200.0
     morbit = make_orbit(scene_center[0], scene_center[1], tc)
     cycle = lines - overlap if n_bursts > 1 else lines
@@ -346,7 +349,7 @@ def make_pair(name, lines, pixels, n_bursts=1, overlap=100, seed=20260101, strea
     sorbit = make_orbit(scene_center[0], scene_center[1], tc + 0.2 * AZ_TIME_INTERVAL,
                         offset=80.0 * morbit.e1 + 90.0 * np.cross(morbit.e2, morbit.e1))
     slave = _swath(seed + 1, lines, pixels, n_bursts, t_first, cycle, slr0, dc_slope_total, sorbit,
-                   scene_center)
+                   scene_center, dc0=slave_dc0)
     del up
     return Pair(name=name, seed=seed, stream=stream, master=master, slave=slave, morbit=morbit,
                 sorbit=sorbit, az_off=np.asarray(az_off, float), rg_off=np.asarray(rg_off, float),
