@@ -58,6 +58,8 @@ int fill_params(tops_plan* p)
         r.kaz = p->ctx->d_kaz;
         r.krg = p->ctx->d_krg;
         r.mom = p->ctx->d_mom;
+        r.kaz_sh = p->ctx->d_kaz_sh;
+        r.krg64 = p->ctx->d_krg64;
         r.out = it.out;
     }
     TOPS_HIP(hipMalloc(&p->d_params, sizeof(ResampleParams) * prm.size()));

@@ -320,25 +320,12 @@ struct ColumnWalker {
     }
 };
 
-template <int O>
-__device__ inline float2 az_dot(const float2 (&w)[WK_WIN], const float (&kl)[kTaps])
-{
-    float r = 0.f, i = 0.f;
-#pragma unroll
-    for (int j = 0; j < kTaps; j++) {
-        r = fmaf(kl[j], w[O + j].x, r);
-        i = fmaf(kl[j], w[O + j].y, i);
-    }
-    return make_float2(r, i);
-}
-
 // One strip (sx, sy) of one burst.  p may live in global memory (batched
 // launches): its fields are wave-uniform.
 struct WalkLds {
     float2 u[WK_KEYS][64];      // azimuth sums per key and column
     double pu[WK_KEYS][64];     // azimuth-interpolated phase per key and column
-    float2 win[WK_WIN][64];     // register windows, for rows served directly
-    double abc[3][64];          // per-column phase coefficients a, b, c
+    double rowt[64][6];         // row-uniform position terms of 64 rows
 };
 
 __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, int sy,
@@ -351,9 +338,10 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     const int c = c0 + lane;
     const bool out_lane = lane < WK_OUT && c < p.m_pixels;
 
-    // column terms of the two position polynomials: same operations, same
-    // order as slave_pos (contraction off), with the row and column
-    // products hoisted
+    // Position polynomial (polyval6 order, contraction off), split into
+    //   ys = ((((A + c2 Pn) + B) + D Pn) + c5 Pn Pn) + AzShift
+    // with A = c0 + c1 Ln, B = c3 Ln Ln, D = c4 Ln uniform over a row and
+    // c2 Pn, c5 Pn Pn fixed per lane: the same operations in the same order.
     double pn, t2a, t5a, t2r, t5r;
     {
 #pragma clang fp contract(off)
@@ -387,15 +375,32 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     int wbase = INT_MIN;   // absolute slave line held in win[0]
 
     for (int r = r0; r < r1; r++) {
-            // ---- this lane's output position (slave_pos, hoisted form)
+        // ---- row-uniform position terms, 64 rows at a time (one per lane)
+        const int rr = (r - r0) & 63;
+        if (rr == 0) {
+            __syncthreads();
+            const int rl = r + lane;
+            if (rl < r1) {
+#pragma clang fp contract(off)
+                const double ln = norm_warp((double)(p.mb[2] + rl), (double)p.mb[2],
+                                            (double)p.mb[3]);
+                sm.rowt[lane][0] = p.cpm_az[0] + p.cpm_az[1] * ln;
+                sm.rowt[lane][1] = p.cpm_az[3] * ln * ln;
+                sm.rowt[lane][2] = p.cpm_az[4] * ln;
+                sm.rowt[lane][3] = p.cpm_rg[0] + p.cpm_rg[1] * ln;
+                sm.rowt[lane][4] = p.cpm_rg[3] * ln * ln;
+                sm.rowt[lane][5] = p.cpm_rg[4] * ln;
+            }
+            __syncthreads();
+        }
+
+        // ---- this lane's output position
         int iy = 0, fy = 0, ix = 0, fx = 0, reg = -1;
         {
 #pragma clang fp contract(off)
-            double ln = norm_warp((double)(p.mb[2] + r), (double)p.mb[2], (double)p.mb[3]);
-            double ys = p.cpm_az[0] + p.cpm_az[1] * ln + t2a + p.cpm_az[3] * ln * ln +
-                        p.cpm_az[4] * ln * pn + t5a + p.az_shift;
-            double xs = p.cpm_rg[0] + p.cpm_rg[1] * ln + t2r + p.cpm_rg[3] * ln * ln +
-                        p.cpm_rg[4] * ln * pn + t5r;
+            const double* t = sm.rowt[rr];
+            double ys = t[0] + t2a + t[1] + t[2] * pn + t5a + p.az_shift;
+            double xs = t[3] + t2r + t[4] + t[5] * pn + t5r;
             ys = fmin(fmax(ys, -1.0e9), 1.0e9);
             xs = fmin(fmax(xs, -1.0e9), 1.0e9);
             iy = (int)trunc(ys);
@@ -408,22 +413,27 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
         const bool servable = reg == 2 && base >= 0 && base <= 64 - kTaps;
         const int key = servable ? iy * kLutRows + fy : INT_MAX;
 
+        tops_cf32 o = {0.0f, 0.0f};
+        if (reg == 1) {
+            tops_ci16 v = p.slave[(size_t)(iy - p.sb[2]) * p.s_pixels + (ix - p.sb[0])];
+            o.x = (float)v.re;
+            o.y = (float)v.im;
+        }
+        bool need = reg == 2;   // interior output not formed yet
+
         // ---- distinct azimuth keys of the row.  Keys change slowly along
         // the row, so a key is listed at each lane where it differs from the
         // lane to its left: one neighbour exchange and a ballot, then a
-        // scalar walk over the set bits (a key that reappears further right
-        // is recognised and not computed twice).
+        // scalar walk over the set bits.
         const int kleft = __shfl_up(key, 1, 64);
         const unsigned long long heads = __ballot(key != INT_MAX && (lane == 0 || kleft != key));
-        int mybuf = -1;
-        const bool has_window = heads != 0ull;
-        if (has_window) {
+        if (heads != 0ull) {
             int iy_min = INT_MAX;
             for (unsigned long long h = heads; h; h &= h - 1)
                 iy_min = min(iy_min, __builtin_amdgcn_readlane(key, __builtin_ctzll(h)) / kLutRows);
             // window: rows iy_min-5 .. iy_min+7 must be held
-            int o = wbase == INT_MIN ? -1 : iy_min - kHalf - wbase;
-            if (o < 0 || o > 64) {
+            int o_win = wbase == INT_MIN ? -1 : iy_min - kHalf - wbase;
+            if (o_win < 0 || o_win > 64) {
                 wbase = iy_min - kHalf;
                 w.anchor(p, wbase);
                 int raw[WK_WIN];
@@ -432,123 +442,93 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
 #pragma unroll
                 for (int k = 0; k < WK_WIN; k++) win[k] = w.deramp(raw[k]);
                 w.prefetch(p);
-                o = 0;
+                o_win = 0;
             }
-            while (o > 3) {   // slide by 4 rows
+            while (o_win > 3) {   // slide by 4 rows
 #pragma unroll
                 for (int k = 0; k < WK_WIN - 4; k++) win[k] = win[k + 4];
 #pragma unroll
                 for (int k = 0; k < 4; k++) win[WK_WIN - 4 + k] = w.deramp(w.pend[k]);
                 w.prefetch(p);
                 wbase += 4;
-                o -= 4;
+                o_win -= 4;
             }
+            // rounds of up to WK_KEYS keys (one round unless the row segment
+            // holds more keys); a key whose rows lie beyond the window's
+            // reach (offset > 4) is left to the direct path
             static_assert(WK_KEYS == 4, "served-key registers below");
-            int sv0 = INT_MAX, sv1 = INT_MAX, sv2 = INT_MAX;   // keys served so far
-            int nk = 0;
-            for (unsigned long long h = heads; h && nk < WK_KEYS; h &= h - 1) {
-                const int kq = __builtin_amdgcn_readlane(key, __builtin_ctzll(h));
-                const bool dup = kq == sv0 || kq == sv1 || kq == sv2;
-                const int iyq = kq / kLutRows, fyq = kq - iyq * kLutRows;
-                const int oq = o + (iyq - iy_min);
-                if (dup || oq > 4) continue;
-                float kl[kTaps];
-                load_taps(p.kaz, fyq, kl);
-                float2 u;
-                switch (oq) {
-                    case 0: u = az_dot<0>(win, kl); break;
-                    case 1: u = az_dot<1>(win, kl); break;
-                    case 2: u = az_dot<2>(win, kl); break;
-                    case 3: u = az_dot<3>(win, kl); break;
-                    default: u = az_dot<4>(win, kl); break;
-                }
-                double4 m = reinterpret_cast<const double4*>(p.mom)[fyq];
-                double y0 = (double)(iyq - p.first_line);
-                double bq = fma(y0, m.x, m.y);
-                double cq = y0 * y0 * m.x + 2.0 * y0 * m.y + m.z;
-                sm.u[nk][lane] = u;
-                sm.pu[nk][lane] = fma(m.x, w.a, fma(bq, w.b, cq * w.c));
-                if (key == kq) mybuf = nk;
-                if (nk == 0) sv0 = kq;
-                else if (nk == 1) sv1 = kq;
-                else if (nk == 2) sv2 = kq;
-                nk++;
-            }
-        }
-        __syncthreads();
-
-        // ---- rows with more azimuth keys than the shared pass takes: the
-        // wave's register windows go to LDS and those outputs do their own
-        // 12x12 sum from there
-        const int orow = iy - kHalf - wbase;
-        const bool lds_direct = servable && mybuf < 0 && has_window && orow >= 0 && orow <= 4;
-        if (__ballot(lds_direct) != 0) {
+            unsigned long long h = heads;
+            do {
+                int sv0 = INT_MAX, sv1 = INT_MAX, sv2 = INT_MAX;   // keys of this round
+                int nk = 0, mybuf = -1;
+                for (; h && nk < WK_KEYS; h &= h - 1) {
+                    const int kq = __builtin_amdgcn_readlane(key, __builtin_ctzll(h));
+                    if (kq == sv0 || kq == sv1 || kq == sv2) continue;
+                    const int iyq = kq / kLutRows, fyq = kq - iyq * kLutRows;
+                    const int oq = o_win + (iyq - iy_min);
+                    if (oq > 4) continue;
+                    // the key's 12 taps placed at window rows oq .. oq+11 (zero
+                    // weights elsewhere add exactly nothing, so the sum equals
+                    // the 12-term sum in the same order)
+                    const float4* kq4 = reinterpret_cast<const float4*>(
+                        p.kaz_sh + ((size_t)oq * kLutRows + fyq) * kAzShiftTaps);
+                    float2 u = make_float2(0.f, 0.f);
 #pragma unroll
-            for (int k = 0; k < WK_WIN; k++) sm.win[k][lane] = win[k];
-            sm.abc[0][lane] = w.a;
-            sm.abc[1][lane] = w.b;
-            sm.abc[2][lane] = w.c;
-            __syncthreads();
-        }
-
-        // ---- outputs
-        if (out_lane) {
-            tops_cf32 o = {0.0f, 0.0f};
-            if (reg == 1) {
-                tops_ci16 s = p.slave[(size_t)(iy - p.sb[2]) * p.s_pixels + (ix - p.sb[0])];
-                o.x = (float)s.re;
-                o.y = (float)s.im;
-            } else if (reg == 2 && mybuf >= 0) {
-                float kp[kTaps];
-                load_taps(p.krg, fx, kp);
-                const float2* u = &sm.u[mybuf][base];
-                const double* pu = &sm.pu[mybuf][base];
-                float sr = 0.f, si = 0.f;
-                double ph = 0.0;
+                    for (int q = 0; q < kAzShiftTaps / 4; q++) {
+                        const float4 k4 = kq4[q];
+                        const float kk[4] = {k4.x, k4.y, k4.z, k4.w};
 #pragma unroll
-                for (int i = 0; i < kTaps; i++) {
-                    float2 v = u[i];
-                    sr = fmaf(kp[i], v.x, sr);
-                    si = fmaf(kp[i], v.y, si);
-                    ph = fma((double)kp[i], pu[i], ph);
-                }
-                o = reramp(sr, si, ph);
-            } else if (lds_direct) {
-                // same operations, same order as the shared pass: azimuth
-                // sum per column, then the range taps over U and PU
-                float kl[kTaps], kp[kTaps];
-                load_taps(p.kaz, fy, kl);
-                load_taps(p.krg, fx, kp);
-                double4 m = reinterpret_cast<const double4*>(p.mom)[fy];
-                double y0 = (double)(iy - p.first_line);
-                double bq = fma(y0, m.x, m.y);
-                double cq = y0 * y0 * m.x + 2.0 * y0 * m.y + m.z;
-                float sr = 0.f, si = 0.f;
-                double ph = 0.0;
-#pragma unroll 2
-                for (int i = 0; i < kTaps; i++) {
-                    const int col = base + i;
-                    float ur = 0.f, ui = 0.f;
-#pragma unroll
-                    for (int j = 0; j < kTaps; j++) {
-                        float2 v = sm.win[orow + j][col];
-                        ur = fmaf(kl[j], v.x, ur);
-                        ui = fmaf(kl[j], v.y, ui);
+                        for (int t = 0; t < 4; t++) {
+                            u.x = fmaf(kk[t], win[4 * q + t].x, u.x);
+                            u.y = fmaf(kk[t], win[4 * q + t].y, u.y);
+                        }
                     }
-                    sr = fmaf(kp[i], ur, sr);
-                    si = fmaf(kp[i], ui, si);
-                    double pu = fma(m.x, sm.abc[0][col], fma(bq, sm.abc[1][col], cq * sm.abc[2][col]));
-                    ph = fma((double)kp[i], pu, ph);
+                    double4 m = reinterpret_cast<const double4*>(p.mom)[fyq];
+                    double y0 = (double)(iyq - p.first_line);
+                    double bq = fma(y0, m.x, m.y);
+                    double cq = y0 * y0 * m.x + 2.0 * y0 * m.y + m.z;
+                    sm.u[nk][lane] = u;
+                    sm.pu[nk][lane] = fma(m.x, w.a, fma(bq, w.b, cq * w.c));
+                    if (key == kq) mybuf = nk;
+                    if (nk == 0) sv0 = kq;
+                    else if (nk == 1) sv1 = kq;
+                    else if (nk == 2) sv2 = kq;
+                    nk++;
                 }
-                o = reramp(sr, si, ph);
-            } else if (reg == 2) {
-                o = interior_direct(p.slave, p.kt, p.tref, p.fdc, p.mom, p.kaz, p.krg,
-                                    p.s_pixels, p.sb[0], p.sb[2], p.first_line, p.dt, iy, fy,
-                                    ix, fx);
-            }
-            p.out[(size_t)r * p.m_pixels + c] = o;
+                __syncthreads();
+                if (need && mybuf >= 0) {
+                    // this output's 12 range taps over U and PU
+                    float kp[kTaps];
+                    load_taps(p.krg, fx, kp);
+                    const double2* kd = reinterpret_cast<const double2*>(p.krg64 + fx * kTaps);
+                    const float2* u = &sm.u[mybuf][base];
+                    const double* pu = &sm.pu[mybuf][base];
+                    float sr = 0.f, si = 0.f;
+                    double ph = 0.0;
+#pragma unroll
+                    for (int i = 0; i < kTaps; i += 2) {
+                        const double2 k2 = kd[i / 2];
+                        float2 v0 = u[i], v1 = u[i + 1];
+                        sr = fmaf(kp[i], v0.x, sr);
+                        si = fmaf(kp[i], v0.y, si);
+                        ph = fma(k2.x, pu[i], ph);
+                        sr = fmaf(kp[i + 1], v1.x, sr);
+                        si = fmaf(kp[i + 1], v1.y, si);
+                        ph = fma(k2.y, pu[i + 1], ph);
+                    }
+                    o = reramp(sr, si, ph);
+                    need = false;
+                }
+                __syncthreads();   // LDS is rewritten by the next round or row
+            } while (h != 0ull);
         }
-        __syncthreads();   // LDS is rewritten by the next row
+
+        // ---- interior outputs the shared pass did not serve
+        if (need) {
+            o = interior_direct(p.slave, p.kt, p.tref, p.fdc, p.mom, p.kaz, p.krg, p.s_pixels,
+                                p.sb[0], p.sb[2], p.first_line, p.dt, iy, fy, ix, fx);
+        }
+        if (out_lane) p.out[(size_t)r * p.m_pixels + c] = o;
     }
 }
 

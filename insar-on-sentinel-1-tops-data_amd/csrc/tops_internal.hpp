@@ -16,6 +16,8 @@ constexpr double kSol = 299792458.0;              // cuConstants.cuh:7
 constexpr double kEllA = 6378137.0;               // cuConstants.cuh:5
 constexpr double kEllB = 6356752.3142451794975639665996337;  // cuConstants.cuh:6
 constexpr int kLutRows = 2048;                    // Interval + 1, Constants.h:9
+constexpr int kAzShifts = 5;     // window offsets the shared azimuth pass takes
+constexpr int kAzShiftTaps = 16; // taps per shifted azimuth row (the window height)
 constexpr int kInterval = 2047;
 constexpr int kTaps = 12;                         // Main.cpp:75, 113
 constexpr int kHalf = kTaps / 2 - 1;              // c_Npoints2m1
@@ -44,6 +46,8 @@ struct tops_ctx {
     float* d_kaz = nullptr;     // [2048 x 12] azimuth LUT
     float* d_krg = nullptr;     // [2048 x 12] range LUT
     double* d_mom = nullptr;    // [2048 x 4] azimuth LUT moments M0 M1 M2 0
+    float* d_kaz_sh = nullptr;  // [5 x 2048 x 16] azimuth rows shifted by 0..4 taps in 16
+    double* d_krg64 = nullptr;  // [2048 x 12] range LUT as double (exact)
     int npoints = 12;
     tops::Scratch scratch;      // FP64 baseline intermediates
 };
@@ -85,6 +89,8 @@ struct ResampleParams {
     const float* kaz;
     const float* krg;
     const double* mom;
+    const float* kaz_sh;    // tops_ctx::d_kaz_sh
+    const double* krg64;    // tops_ctx::d_krg64
     tops_cf32* out;
 };
 

@@ -105,6 +105,25 @@ def test_resample_shift_and_offset_boxes(orc, c1, rs):
     assert e <= TOL_RESAMPLE
 
 
+@pytest.mark.parametrize("shear", [0.02, 0.5, 40.0])
+def test_resample_azimuth_shear(orc, c1, rs, shear):
+    """An azimuth offset that varies along range: at 0.02 px per unit of Pn
+    a 50-column row segment holds a handful of azimuth keys, at 0.5 nearly
+    every output has its own key (the shared pass runs in several rounds),
+    at 40 the keys span more lines than the window reaches and those outputs
+    take the direct 144-tap path."""
+    L, P = c1["L"], c1["P"]
+    box = [0, P - 1, 0, L - 1]
+    ca = np.array(c1["ca"], dtype=float)
+    ca[2] += shear
+    ref = _oracle(orc, c1, box, box, 0, 0.0, ca, c1["cr"])
+    got, _ = rs.run(c1["S"], box, box, 0, c1["dt"], 0.0, ca, c1["cr"], c1["kt"], c1["tref"],
+                    c1["fdc"], 0)
+    e = rel_err(got, ref)
+    print(f"  azimuth shear {shear}: max|dz|/rms = {e:.3e}")
+    assert e <= TOL_RESAMPLE
+
+
 def test_resample_out_of_box_is_zero(orc, c1, rs):
     """Offsets pushing part of the burst outside the slave box: zero region
     and edge band must match the oracle exactly in structure."""
