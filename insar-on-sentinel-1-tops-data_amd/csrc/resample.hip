@@ -320,6 +320,10 @@ struct ColumnWalker {
     }
 };
 
+// read-only tables addressed with wave-uniform indices
+typedef __attribute__((address_space(4))) const float CFloat;
+typedef __attribute__((address_space(4))) const double CDouble;
+
 // One strip (sx, sy) of one burst.  p may live in global memory (batched
 // launches): its fields are wave-uniform.
 struct WalkLds {
@@ -470,20 +474,19 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
                     // the key's 12 taps placed at window rows oq .. oq+11 (zero
                     // weights elsewhere add exactly nothing, so the sum equals
                     // the 12-term sum in the same order)
-                    const float4* kq4 = reinterpret_cast<const float4*>(
-                        p.kaz_sh + ((size_t)oq * kLutRows + fyq) * kAzShiftTaps);
+                    // uniform addresses: read through the constant address
+                    // space so they become scalar loads
+                    CFloat* taps =
+                        (CFloat*)(p.kaz_sh + ((size_t)oq * kLutRows + fyq) * kAzShiftTaps);
                     float2 u = make_float2(0.f, 0.f);
 #pragma unroll
-                    for (int q = 0; q < kAzShiftTaps / 4; q++) {
-                        const float4 k4 = kq4[q];
-                        const float kk[4] = {k4.x, k4.y, k4.z, k4.w};
-#pragma unroll
-                        for (int t = 0; t < 4; t++) {
-                            u.x = fmaf(kk[t], win[4 * q + t].x, u.x);
-                            u.y = fmaf(kk[t], win[4 * q + t].y, u.y);
-                        }
+                    for (int t = 0; t < kAzShiftTaps; t++) {
+                        const float k = taps[t];
+                        u.x = fmaf(k, win[t].x, u.x);
+                        u.y = fmaf(k, win[t].y, u.y);
                     }
-                    double4 m = reinterpret_cast<const double4*>(p.mom)[fyq];
+                    CDouble* mq = (CDouble*)(p.mom + 4 * fyq);
+                    const double4 m = make_double4(mq[0], mq[1], mq[2], 0.0);
                     double y0 = (double)(iyq - p.first_line);
                     double bq = fma(y0, m.x, m.y);
                     double cq = y0 * y0 * m.x + 2.0 * y0 * m.y + m.z;
