@@ -66,7 +66,7 @@ __device__ __forceinline__ void coherence_tile(const tops_ci16* M, const tops_cf
         size_t q = (size_t)r * w + c;
         tops_cf32 s = S[q];
         float out = 0.0f;
-        if (sqrtf(s.x * s.x + s.y * s.y) >= 0.001f) out = ws[k].coherence();
+        if (amplitude_ok(s.x * s.x + s.y * s.y)) out = ws[k].coherence();
         coh[q] = out;
     }
 }

@@ -17,6 +17,7 @@
 #pragma once
 
 #include "tops_internal.hpp"
+#include "tile_window.hpp"
 
 namespace tops {
 
@@ -121,7 +122,7 @@ struct CohWalk {
                 unsigned f = 0;
 #pragma unroll
                 for (int j = 0; j < CW_COLS; j++)
-                    f |= (sqrtf(v[DX + j].z) >= 0.001f ? 1u : 0u) << j;
+                    f |= (amplitude_ok(v[DX + j].z) ? 1u : 0u) << j;
                 ok[q] = f;
                 if (gr - first < WA - 1) continue;
                 // output row r: input rows r-DY .. r-DY+WA-1 are in the buffer
@@ -133,9 +134,7 @@ struct CohWalk {
                     float4 a = H[0][j];
 #pragma unroll
                     for (int t = 1; t < WA; t++) a = cw_add(a, H[t][j]);
-                    const float pp = a.z * a.w;
-                    float c = pp > 0.0f ? sqrtf(a.x * a.x + a.y * a.y) / sqrtf(pp) : 0.0f;
-                    out[j] = (ok[slot_r] >> j) & 1u ? c : 0.0f;
+                    out[j] = (ok[slot_r] >> j) & 1u ? coherence_of(a.x, a.y, a.z, a.w) : 0.0f;
                 }
                 float* dst = coh + (size_t)r * w + c0;
                 if (vec && c0 + CW_COLS <= w) {

@@ -254,7 +254,20 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
             fclose(f);
         }
     }
-    const int cand[] = {32, 64, 128, 256, 512};
+    // fixed heights, plus the heights that fill 1, 2 and 3 rounds of the
+    // GPU's walker wave slots with no partial round
+    std::vector<int> cand = {32, 64, 128, 256, 512};
+    {
+        std::vector<int> ls, ps;
+        for (auto& it : p->items) {
+            ls.push_back(it.geom.master_box[3] - it.geom.master_box[2] + 1);
+            ps.push_back(it.geom.master_box[1] - it.geom.master_box[0] + 1);
+        }
+        for (int r = 1; r <= 3; r++) {
+            int h = std::max(16, resample_rows_for_rounds(ls.data(), ps.data(), (int)ls.size(), r));
+            if (std::find(cand.begin(), cand.end(), h) == cand.end()) cand.push_back(h);
+        }
+    }
     float best = 1e30f;
     int best_r = p->strip_rows;
     hipEvent_t e0, e1;

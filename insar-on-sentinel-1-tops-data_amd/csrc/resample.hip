@@ -35,6 +35,7 @@
 // + M2) c exactly, so Phi needs three 12-tap FP64 range dot products.
 #include "tops_internal.hpp"
 
+#include <algorithm>
 #include <climits>
 
 namespace tops {
@@ -189,7 +190,6 @@ int launch_resample_fp64(tops_ctx* ctx, const ResampleParams& p, hipStream_t s)
 // WK_KEYS keys in one row segment, or a window outside the strip's 64
 // columns) are computed by interior_direct.
 constexpr int WK_OUT = 50;     // output columns per strip (64-lane wave)
-constexpr int WK_ROWS = 128;   // output rows per strip
 constexpr int WK_WIN = 16;     // register window rows
 constexpr int WK_KEYS = 4;     // azimuth keys per row served by the shared pass
 
@@ -558,9 +558,49 @@ int resample_strip_count(int m_lines, int m_pixels, int strip_rows, int* nx, int
     return *nx * *ny;
 }
 
+// Walker waves the GPU holds at once: CUs x 4 SIMDs x the kernel's waves per
+// SIMD (3, amdgpu_waves_per_eu above).  Queried once per device.
+int resample_wave_slots()
+{
+    static int slots[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256 * 4 * 3;
+    if (!slots[dev]) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        slots[dev] = cus * 4 * 3;
+    }
+    return slots[dev];
+}
+
+// Strip height whose strips (each one wave) fill `rounds` rounds of the
+// GPU's wave slots without a partial extra round: the smallest height with
+// sum_i nx_i * ceil(lines_i / h) <= rounds * slots.
+int resample_rows_for_rounds(const int* lines, const int* pixels, int n, int rounds)
+{
+    const long cap = (long)rounds * resample_wave_slots();
+    int lo = 8, hi = 8;
+    for (int i = 0; i < n; i++) hi = std::max(hi, lines[i]);
+    auto strips = [&](int h) {
+        long t = 0;
+        for (int i = 0; i < n; i++)
+            t += (long)((pixels[i] + WK_OUT - 1) / WK_OUT) * ((lines[i] + h - 1) / h);
+        return t;
+    };
+    if (strips(hi) > cap) return hi;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (strips(mid) <= cap) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
 int launch_resample_fast(const ResampleParams& p, hipStream_t s, int strip_rows)
 {
-    if (strip_rows <= 0) strip_rows = WK_ROWS;
+    if (strip_rows <= 0) strip_rows = resample_rows_for_rounds(&p.m_lines, &p.m_pixels, 1, 1);
     dim3 grid((p.m_pixels + WK_OUT - 1) / WK_OUT, (p.m_lines + strip_rows - 1) / strip_rows);
     hipLaunchKernelGGL(resample_fast_kernel, grid, dim3(64), 0, s, p, strip_rows);
     TOPS_HIP(hipGetLastError());

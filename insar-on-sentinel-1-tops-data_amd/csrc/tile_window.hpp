@@ -33,13 +33,25 @@ inline bool tile_window_fits(int win_rg, int win_az)
     return win_rg >= 1 && win_az >= 1 && tile_window_lds_bytes(win_rg, win_az) <= TW_LDS_LIMIT;
 }
 
+// |sum M S*| / sqrt(sum |S|^2 sum |M|^2), 0 unless the product is positive,
+// with the hardware square root and reciprocal square root (about 1 ulp
+// each; the coherence tolerance is 1e-4 absolute)
+__device__ __forceinline__ float coherence_of(float a0, float a1, float a2, float a3)
+{
+    const float pp = a2 * a3;
+    return pp > 0.0f ? __builtin_amdgcn_sqrtf(a0 * a0 + a1 * a1) * __builtin_amdgcn_rsqf(pp)
+                     : 0.0f;
+}
+
+// |S| >= 0.001 (EstCoh.cu:446-449) from |S|^2
+__device__ __forceinline__ bool amplitude_ok(float s2)
+{
+    return __builtin_amdgcn_sqrtf(s2) >= 0.001f;
+}
+
 struct WindowSums {
     float a0, a1, a2, a3;   // sum Re(M S*), sum Im(M S*), sum |S|^2, sum |M|^2
-    __device__ float coherence() const
-    {
-        float pp = a2 * a3;
-        return pp > 0.0f ? sqrtf(a0 * a0 + a1 * a1) / sqrtf(pp) : 0.0f;
-    }
+    __device__ float coherence() const { return coherence_of(a0, a1, a2, a3); }
 };
 
 __device__ inline void kahan_add(float& s, float& comp, float v)

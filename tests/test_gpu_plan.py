@@ -104,7 +104,8 @@ def test_plan_tune_and_graph_replay(tops, dev, batch):
     best = C.c_int(0)
     print("  plan: tuning", flush=True)
     N.check(L.tops_plan_tune(plan, sp, C.byref(best)), "tops_plan_tune")
-    assert best.value in (32, 64, 128, 256, 512)
+    # fixed candidates or a height that fills whole rounds of wave slots
+    assert 16 <= best.value <= 4096
     assert L.tops_plan_strip_rows(plan) == best.value
     N.check(L.tops_plan_run(plan, sp))
     s.synchronize()
