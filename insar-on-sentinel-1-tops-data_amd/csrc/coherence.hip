@@ -15,6 +15,8 @@
 #include "tile_window.hpp"
 #include "coherence_walk.hpp"
 
+#include <algorithm>
+
 namespace tops {
 
 __global__ void __launch_bounds__(256) coherence_fp64_kernel(const tops_ci16* M, const tops_cf32* S, int w,
@@ -96,28 +98,30 @@ using CohWalk3x10 = CohWalk<10, 3, 4, 1>;
 
 __device__ __forceinline__ void coherence_walk_block(const tops_ci16* M, const tops_cf32* S,
                                                      int w, int h, float* coh, int bx, int by,
-                                                     int vec)
+                                                     int vec, int rpw)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int c0 = CW_COLS * (bx * 64 + lane);
-    const int r0 = (by * 4 + wv) * CW_ROWS;
+    const int r0 = (by * 4 + wv) * rpw;
     if (c0 >= w || r0 >= h) return;
-    CohWalk3x10::walk(M, S, w, h, coh, c0, r0, min(r0 + CW_ROWS, h), vec != 0);
+    CohWalk3x10::walk(M, S, w, h, coh, c0, r0, min(r0 + rpw, h), vec != 0);
 }
 
-__global__ void __launch_bounds__(256) coherence_walk_kernel(const tops_ci16* M, const tops_cf32* S,
-                                                             int w, int h, float* coh, int vec)
+// rpw: output rows per wave walk
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+coherence_walk_kernel(const tops_ci16* M, const tops_cf32* S, int w, int h, float* coh, int vec,
+                      int rpw)
 {
-    coherence_walk_block(M, S, w, h, coh, blockIdx.x, blockIdx.y, vec);
+    coherence_walk_block(M, S, w, h, coh, blockIdx.x, blockIdx.y, vec, rpw);
 }
 
-// Batched form: work[i] = {item, block x, block y, vec}.
-__global__ void __launch_bounds__(256) coherence_walk_batch_kernel(const CohItem* items,
-                                                                   const int4* work)
+// Batched form: work[i] = {item, block x, block y, vec}, CW_ROWS rows per wave.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+coherence_walk_batch_kernel(const CohItem* items, const int4* work)
 {
     const int4 wk = work[blockIdx.x];
     const CohItem& it = items[wk.x];
-    coherence_walk_block(it.m, it.s, it.w, it.h, it.coh, wk.y, wk.z, wk.w);
+    coherence_walk_block(it.m, it.s, it.w, it.h, it.coh, wk.y, wk.z, wk.w, CW_ROWS);
 }
 
 bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az)
@@ -136,6 +140,25 @@ int coherence_walk_count(int w, int h, int* nx, int* ny)
     *nx = (w + 64 * CW_COLS - 1) / (64 * CW_COLS);
     *ny = (h + 4 * CW_ROWS - 1) / (4 * CW_ROWS);
     return *nx * *ny;
+}
+
+// Rows per wave for a single burst such that its waves fit in one round of
+// the GPU's slots (CUs x 4 SIMDs x 3 waves); at least CW_ROWS / 2.
+static int coherence_walk_rows(int w, int h)
+{
+    static int slots = 0;
+    if (!slots) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        slots = cus * 4 * 3;
+    }
+    const int nx = (w + 64 * CW_COLS - 1) / (64 * CW_COLS);
+    const int ny = std::max(1, slots / 4 / nx);
+    const int rows_per_block = (h + ny - 1) / ny;
+    return std::max(CW_ROWS / 2, (rows_per_block + 3) / 4);
 }
 
 int coherence_tile_count(int w, int h, int* nx, int* ny)
@@ -173,10 +196,11 @@ int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h, int d
         hipLaunchKernelGGL(coherence_fp64_kernel, g, dim3(256), 0, st, m, s, w, h, dx, dy,
                            win_rg, win_az, coh);
     } else if (coherence_walk_fits(dx, dy, win_rg, win_az)) {
-        int nx, ny;
-        coherence_walk_count(w, h, &nx, &ny);
+        const int rpw = coherence_walk_rows(w, h);
+        const int nx = (w + 64 * CW_COLS - 1) / (64 * CW_COLS);
+        const int ny = (h + 4 * rpw - 1) / (4 * rpw);
         hipLaunchKernelGGL(coherence_walk_kernel, dim3(nx, ny), dim3(256), 0, st, m, s, w, h, coh,
-                           coherence_walk_vec(m, s, coh, w));
+                           coherence_walk_vec(m, s, coh, w), rpw);
     } else {
         dim3 g((w + TW_COLS - 1) / TW_COLS, (h + TW_ROWS - 1) / TW_ROWS);
         hipLaunchKernelGGL(coherence_fast_kernel, g, dim3(256),

@@ -44,48 +44,72 @@ struct CohWalk {
     static constexpr int OFF = RDX - DX;                // first needed sample in the load
     static constexpr int L = (OFF + N + 3) / 4 * 4;     // samples loaded per row
 
-    // products of the N samples of input row gr, columns c0-DX .. c0-DX+N-1,
-    // zero outside the burst.  vec: w % 4 == 0 and M, S, coh 16-byte aligned
-    __device__ static void row(const tops_ci16* M, const tops_cf32* S, int w, int h, int gr,
-                               int c0, bool vec, float4 (&v)[N])
+    // raw samples of one input row: L master (packed CInt16) and L slave
+    // (CF32) samples from column c0 - RDX on
+    struct Raw {
+        int4 m[L / 4];
+        float4 s[L / 2];
+    };
+
+    // the row's raw samples (zero outside the burst).  vec: w % 4 == 0 and
+    // M, S, coh 16-byte aligned; rows in the middle of the burst then use
+    // 16-byte loads only.
+    __device__ static void fetch(const tops_ci16* M, const tops_cf32* S, int w, int h, int gr,
+                                 int c0, bool vec, Raw& raw)
     {
         if (gr < 0 || gr >= h) {
 #pragma unroll
-            for (int k = 0; k < N; k++) v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int q = 0; q < L / 4; q++) raw.m[q] = make_int4(0, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < L / 2; q++) raw.s[q] = make_float4(0.f, 0.f, 0.f, 0.f);
             return;
         }
         const size_t rb = (size_t)gr * w;
         const int lo = c0 - RDX;
         if (vec && lo >= 0 && lo + L <= w) {
             // aligned: rb + lo is a multiple of 4 samples (w % 4 == 0, c0 % 4 == 0)
-            int m[L];
-            float s[2 * L];
             const int4* mq = reinterpret_cast<const int4*>(M + rb + lo);
             const float4* sq = reinterpret_cast<const float4*>(S + rb + lo);
 #pragma unroll
-            for (int q = 0; q < L / 4; q++) {
-                int4 t = mq[q];
-                m[4 * q] = t.x; m[4 * q + 1] = t.y; m[4 * q + 2] = t.z; m[4 * q + 3] = t.w;
-            }
+            for (int q = 0; q < L / 4; q++) raw.m[q] = mq[q];
 #pragma unroll
-            for (int q = 0; q < L / 2; q++) {
-                float4 t = sq[q];
-                s[4 * q] = t.x; s[4 * q + 1] = t.y; s[4 * q + 2] = t.z; s[4 * q + 3] = t.w;
-            }
-#pragma unroll
-            for (int k = 0; k < N; k++)
-                v[k] = cw_prod(m[OFF + k], s[2 * (OFF + k)], s[2 * (OFF + k) + 1]);
+            for (int q = 0; q < L / 2; q++) raw.s[q] = sq[q];
         } else {
             const int* mi = reinterpret_cast<const int*>(M);
+            int m[L];
+            float sv[2 * L];
 #pragma unroll
-            for (int k = 0; k < N; k++) {
-                const int c = c0 - DX + k;
-                v[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int k = 0; k < L; k++) {
+                const int c = lo + k;
+                m[k] = 0;
+                sv[2 * k] = sv[2 * k + 1] = 0.f;
                 if (c >= 0 && c < w) {
                     const tops_cf32 t = S[rb + c];
-                    v[k] = cw_prod(mi[rb + c], t.x, t.y);
+                    m[k] = mi[rb + c];
+                    sv[2 * k] = t.x;
+                    sv[2 * k + 1] = t.y;
                 }
             }
+#pragma unroll
+            for (int q = 0; q < L / 4; q++)
+                raw.m[q] = make_int4(m[4 * q], m[4 * q + 1], m[4 * q + 2], m[4 * q + 3]);
+#pragma unroll
+            for (int q = 0; q < L / 2; q++)
+                raw.s[q] = make_float4(sv[4 * q], sv[4 * q + 1], sv[4 * q + 2], sv[4 * q + 3]);
+        }
+    }
+
+    // products of the N samples the 4 outputs need (columns c0-DX .. c0-DX+N-1)
+    __device__ static void products(const Raw& raw, float4 (&v)[N])
+    {
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const int e = OFF + k;
+            const int4 mq = raw.m[e / 4];
+            const int mv = (e % 4 == 0) ? mq.x : (e % 4 == 1) ? mq.y : (e % 4 == 2) ? mq.z : mq.w;
+            const float4 sq = raw.s[e / 2];
+            const float sr = (e % 2 == 0) ? sq.x : sq.z, si = (e % 2 == 0) ? sq.y : sq.w;
+            v[k] = cw_prod(mv, sr, si);
         }
     }
 
@@ -111,13 +135,17 @@ struct CohWalk {
         unsigned ok[WA];           // |S| >= 0.001 flags of those rows' own 4 columns
         const int first = r_begin - DY;            // first input row
         const int last = r_end - 1 - DY + WA - 1;  // last input row
+        Raw nxt;                                   // next row, loads in flight
+        fetch(M, S, w, h, first, c0, vec, nxt);
         for (int base = first; base <= last; base += WA) {
 #pragma unroll
             for (int q = 0; q < WA; q++) {
                 const int gr = base + q;
                 if (gr > last) break;
+                const Raw cur = nxt;
+                if (gr < last) fetch(M, S, w, h, gr + 1, c0, vec, nxt);
                 float4 v[N];
-                row(M, S, w, h, gr, c0, vec, v);
+                products(cur, v);
                 range_sums(v, H[q]);
                 unsigned f = 0;
 #pragma unroll
