@@ -432,9 +432,10 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
         const int kleft = __shfl_up(key, 1, 64);
         const unsigned long long heads = __ballot(key != INT_MAX && (lane == 0 || kleft != key));
         if (heads != 0ull) {
+            // keys are iy * 2048 + fy with iy >= 0 (interior rows of the slave box)
             int iy_min = INT_MAX;
             for (unsigned long long h = heads; h; h &= h - 1)
-                iy_min = min(iy_min, __builtin_amdgcn_readlane(key, __builtin_ctzll(h)) / kLutRows);
+                iy_min = min(iy_min, (int)((unsigned)__builtin_amdgcn_readlane(key, __builtin_ctzll(h)) >> 11));
             // window: rows iy_min-5 .. iy_min+7 must be held
             int o_win = wbase == INT_MIN ? -1 : iy_min - kHalf - wbase;
             if (o_win < 0 || o_win > 64) {
@@ -459,16 +460,16 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
             }
             // rounds of up to WK_KEYS keys (one round unless the row segment
             // holds more keys); a key whose rows lie beyond the window's
-            // reach (offset > 4) is left to the direct path
-            static_assert(WK_KEYS == 4, "served-key registers below");
+            // reach (offset > 4) is left to the direct path.  A key that
+            // reappears further along the row takes another slot (its lanes
+            // use the later one): extra work in that rare case, same result.
+            static_assert(kLutRows == 2048, "key packing below");
             unsigned long long h = heads;
             do {
-                int sv0 = INT_MAX, sv1 = INT_MAX, sv2 = INT_MAX;   // keys of this round
                 int nk = 0, mybuf = -1;
                 for (; h && nk < WK_KEYS; h &= h - 1) {
                     const int kq = __builtin_amdgcn_readlane(key, __builtin_ctzll(h));
-                    if (kq == sv0 || kq == sv1 || kq == sv2) continue;
-                    const int iyq = kq / kLutRows, fyq = kq - iyq * kLutRows;
+                    const int iyq = (int)((unsigned)kq >> 11), fyq = kq & (kLutRows - 1);
                     const int oq = o_win + (iyq - iy_min);
                     if (oq > 4) continue;
                     // the key's 12 taps placed at window rows oq .. oq+11 (zero
@@ -493,9 +494,6 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
                     sm.u[nk][lane] = u;
                     sm.pu[nk][lane] = fma(m.x, w.a, fma(bq, w.b, cq * w.c));
                     if (key == kq) mybuf = nk;
-                    if (nk == 0) sv0 = kq;
-                    else if (nk == 1) sv1 = kq;
-                    else if (nk == 2) sv2 = kq;
                     nk++;
                 }
                 __syncthreads();
