@@ -217,46 +217,46 @@ __device__ inline tops_cf32 reramp(float sr, float si, double phi)
 }
 
 // Interior output computed straight from HBM (outputs the shared pass does
-// not serve).  Out of line to keep the walker's register budget; it takes
-// plain values, not the parameter block, so the kernel never needs the
-// block's address (which would put a copy of it in scratch memory).
-__device__ __noinline__ tops_cf32 interior_direct(const tops_ci16* slave, const double* kt,
-                                                 const double* tref, const double* fdc,
-                                                 const double* mom, const float* kaz,
-                                                 const float* krg, int s_pixels, int sx0,
-                                                 int sy0, int first_line, double dt, int iy,
-                                                 int fy, int ix, int fx)
+// not serve).  Inline with rolled loops and one tap loaded per step, so it
+// stays small in registers; it takes plain values, not the parameter block.
+__device__ __forceinline__ tops_cf32 interior_direct(const tops_ci16* slave, const double* kt,
+                                                     const double* tref, const double* fdc,
+                                                     const double* mom, const float* kaz,
+                                                     const float* krg, int s_pixels, int sx0,
+                                                     int sy0, int first_line, double dt, int iy,
+                                                     int fy, int ix, int fx)
 {
-    float kl[kTaps], kp[kTaps];
-    load_taps(kaz, fy, kl);
-    load_taps(krg, fx, kp);
     float sr = 0.f, si = 0.f;
     double A = 0.0, B = 0.0, C = 0.0;
+#pragma unroll 1
     for (int j = 0; j < kTaps; j++) {
-        int ya = iy - kHalf + j;
+        const int ya = iy - kHalf + j;
+        const double y = (double)(ya - first_line);
         float tr = 0.f, ti = 0.f;
+#pragma unroll 1
         for (int i = 0; i < kTaps; i++) {
-            int xa = ix - kHalf + i;
+            const int xa = ix - kHalf + i;
+            const float kpi = krg[fx * kTaps + i];
             double k = kt[xa], t = tref[xa], f = fdc[xa];
             double a = -kPi * k * (t * t);
             double b = 2.0 * kPi * dt * (k * t - f);
             double c = -kPi * k * (dt * dt);
-            double y = (double)(ya - first_line);
             float sn, cs;
             sincosf(reduce_2pi(fma(y, fma(y, c, b), a)), &sn, &cs);
             tops_ci16 v = slave[(size_t)(ya - sy0) * s_pixels + (xa - sx0)];
             float dr = (float)v.re * cs - (float)v.im * sn;
             float di = (float)v.re * sn + (float)v.im * cs;
-            tr = fmaf(kp[i], dr, tr);
-            ti = fmaf(kp[i], di, ti);
+            tr = fmaf(kpi, dr, tr);
+            ti = fmaf(kpi, di, ti);
             if (j == 0) {
-                A = fma((double)kp[i], a, A);
-                B = fma((double)kp[i], b, B);
-                C = fma((double)kp[i], c, C);
+                A = fma((double)kpi, a, A);
+                B = fma((double)kpi, b, B);
+                C = fma((double)kpi, c, C);
             }
         }
-        sr = fmaf(kl[j], tr, sr);
-        si = fmaf(kl[j], ti, si);
+        const float klj = kaz[fy * kTaps + j];
+        sr = fmaf(klj, tr, sr);
+        si = fmaf(klj, ti, si);
     }
     double4 m = reinterpret_cast<const double4*>(mom)[fy];
     double y0 = (double)(iy - first_line);
@@ -464,8 +464,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
             // reappears further along the row takes another slot (its lanes
             // use the later one): extra work in that rare case, same result.
             static_assert(kLutRows == 2048, "key packing below");
-            unsigned long long h = heads;
-            do {
+            auto serve_round = [&](unsigned long long& h) {
                 int nk = 0, mybuf = -1;
                 for (; h && nk < WK_KEYS; h &= h - 1) {
                     const int kq = __builtin_amdgcn_readlane(key, __builtin_ctzll(h));
@@ -521,7 +520,10 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
                     need = false;
                 }
                 __syncthreads();   // LDS is rewritten by the next round or row
-            } while (h != 0ull);
+            };
+            unsigned long long h = heads;
+            serve_round(h);                  // the common case: one round
+            while (h != 0ull) serve_round(h);
         }
 
         // ---- interior outputs the shared pass did not serve
