@@ -192,6 +192,7 @@ int launch_resample_fp64(tops_ctx* ctx, const ResampleParams& p, hipStream_t s)
 constexpr int WK_OUT = 50;     // output columns per strip (64-lane wave)
 constexpr int WK_WIN = 16;     // register window rows
 constexpr int WK_KEYS = 4;     // azimuth keys per row served by the shared pass
+constexpr int kWalkWavesPerSimd = 4;   // occupancy the walker is compiled for
 
 __device__ inline void load_taps(const float* lut, int row, float k[kTaps])
 {
@@ -269,7 +270,6 @@ __device__ __forceinline__ tops_cf32 interior_direct(const tops_ci16* slave, con
 struct ColumnWalker {
     int xcol;            // absolute slave column
     bool in_box;         // column inside the slave box
-    double a, b, c;      // phi(y) = a + b y + c y^2, y burst-relative line
     double pr, pi;       // exp(j phi(ynext))
     double ir, ii;       // exp(j (b + c (2 ynext + 1)))
     double qr, qi;       // exp(j 2c)
@@ -285,7 +285,8 @@ struct ColumnWalker {
                                                      (xcol - p.sb[0])];
     }
 
-    __device__ void anchor(const ResampleParams& p, int y_abs)
+    // phi(y) = a + b y + c y^2 (y burst-relative line), anchored at y_abs
+    __device__ void anchor(const ResampleParams& p, int y_abs, double a, double b, double c)
     {
         ynext = y_abs;
         double y = (double)(y_abs - p.first_line);
@@ -329,7 +330,9 @@ typedef __attribute__((address_space(4))) const double CDouble;
 struct WalkLds {
     float2 u[WK_KEYS][64];      // azimuth sums per key and column
     double pu[WK_KEYS][64];     // azimuth-interpolated phase per key and column
-    double rowt[64][6];         // row-uniform position terms of 64 rows
+    double rowt[32][6];         // row-uniform position terms of 32 rows
+    double pos[5][64];          // per lane: Pn, c2 Pn and c5 Pn Pn of both polynomials
+    double abc[3][64];          // per lane: the column's phase coefficients a, b, c
 };
 
 __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, int sy,
@@ -346,15 +349,16 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     //   ys = ((((A + c2 Pn) + B) + D Pn) + c5 Pn Pn) + AzShift
     // with A = c0 + c1 Ln, B = c3 Ln Ln, D = c4 Ln uniform over a row and
     // c2 Pn, c5 Pn Pn fixed per lane: the same operations in the same order.
-    double pn, t2a, t5a, t2r, t5r;
+    // (kept in LDS, read once per row: registers are the walker's limit)
     {
 #pragma clang fp contract(off)
-        pn = norm_warp((double)(p.mb[0] + min(c, p.m_pixels - 1)), (double)p.mb[0],
-                       (double)p.mb[1]);
-        t2a = p.cpm_az[2] * pn;
-        t5a = p.cpm_az[5] * pn * pn;
-        t2r = p.cpm_rg[2] * pn;
-        t5r = p.cpm_rg[5] * pn * pn;
+        const double pn = norm_warp((double)(p.mb[0] + min(c, p.m_pixels - 1)),
+                                    (double)p.mb[0], (double)p.mb[1]);
+        sm.pos[0][lane] = pn;
+        sm.pos[1][lane] = p.cpm_az[2] * pn;
+        sm.pos[2][lane] = p.cpm_az[5] * pn * pn;
+        sm.pos[3][lane] = p.cpm_rg[2] * pn;
+        sm.pos[4][lane] = p.cpm_rg[5] * pn * pn;
     }
 
     // slave column of this lane
@@ -366,11 +370,16 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     ColumnWalker w;
     w.xcol = ucol0 + lane;
     w.in_box = w.xcol >= p.sb[0] && w.xcol <= p.sb[1];
-    w.a = w.b = w.c = 0.0;
-    if (w.in_box) phase_coeffs(p, w.xcol, w.a, w.b, w.c);
+    {
+        double a = 0.0, b = 0.0, cc = 0.0;
+        if (w.in_box) phase_coeffs(p, w.xcol, a, b, cc);
+        sm.abc[0][lane] = a;
+        sm.abc[1][lane] = b;
+        sm.abc[2][lane] = cc;
+    }
     {
         double s, cc;
-        double d = 2.0 * w.c;
+        double d = 2.0 * sm.abc[2][lane];
         sincos(fma(-rint(d * kInvTwoPi), kTwoPi, d), &s, &cc);
         w.qr = cc; w.qi = s;
     }
@@ -380,11 +389,11 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
 
     for (int r = r0; r < r1; r++) {
         // ---- row-uniform position terms, 64 rows at a time (one per lane)
-        const int rr = (r - r0) & 63;
+        const int rr = (r - r0) & 31;
         if (rr == 0) {
             __syncthreads();
             const int rl = r + lane;
-            if (rl < r1) {
+            if (lane < 32 && rl < r1) {
 #pragma clang fp contract(off)
                 const double ln = norm_warp((double)(p.mb[2] + rl), (double)p.mb[2],
                                             (double)p.mb[3]);
@@ -403,8 +412,9 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
         {
 #pragma clang fp contract(off)
             const double* t = sm.rowt[rr];
-            double ys = t[0] + t2a + t[1] + t[2] * pn + t5a + p.az_shift;
-            double xs = t[3] + t2r + t[4] + t[5] * pn + t5r;
+            const double pn = sm.pos[0][lane];
+            double ys = t[0] + sm.pos[1][lane] + t[1] + t[2] * pn + sm.pos[2][lane] + p.az_shift;
+            double xs = t[3] + sm.pos[3][lane] + t[4] + t[5] * pn + sm.pos[4][lane];
             ys = fmin(fmax(ys, -1.0e9), 1.0e9);
             xs = fmin(fmax(xs, -1.0e9), 1.0e9);
             iy = (int)trunc(ys);
@@ -440,7 +450,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
             int o_win = wbase == INT_MIN ? -1 : iy_min - kHalf - wbase;
             if (o_win < 0 || o_win > 64) {
                 wbase = iy_min - kHalf;
-                w.anchor(p, wbase);
+                w.anchor(p, wbase, sm.abc[0][lane], sm.abc[1][lane], sm.abc[2][lane]);
                 int raw[WK_WIN];
 #pragma unroll
                 for (int k = 0; k < WK_WIN; k++) raw[k] = w.fetch(p, wbase + k);
@@ -491,7 +501,8 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
                     double bq = fma(y0, m.x, m.y);
                     double cq = y0 * y0 * m.x + 2.0 * y0 * m.y + m.z;
                     sm.u[nk][lane] = u;
-                    sm.pu[nk][lane] = fma(m.x, w.a, fma(bq, w.b, cq * w.c));
+                    sm.pu[nk][lane] = fma(m.x, sm.abc[0][lane],
+                                          fma(bq, sm.abc[1][lane], cq * sm.abc[2][lane]));
                     if (key == kq) mybuf = nk;
                     nk++;
                 }
@@ -535,7 +546,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     }
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) resample_fast_kernel(ResampleParams p, int strip_rows)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWalkWavesPerSimd))) resample_fast_kernel(ResampleParams p, int strip_rows)
 {
     __shared__ WalkLds sm;
     resample_strip(p, blockIdx.x, blockIdx.y, strip_rows, sm);
@@ -543,7 +554,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) re
 
 // Batched form: one workgroup per entry of a strip table covering many
 // bursts (work[i] = {burst, strip x, strip y, strip rows}).
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) resample_batch_kernel(const ResampleParams* items,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWalkWavesPerSimd))) resample_batch_kernel(const ResampleParams* items,
                                                             const int4* work)
 {
     __shared__ WalkLds sm;
@@ -559,18 +570,18 @@ int resample_strip_count(int m_lines, int m_pixels, int strip_rows, int* nx, int
 }
 
 // Walker waves the GPU holds at once: CUs x 4 SIMDs x the kernel's waves per
-// SIMD (3, amdgpu_waves_per_eu above).  Queried once per device.
+// SIMD (kWalkWavesPerSimd).  Queried once per device.
 int resample_wave_slots()
 {
     static int slots[64] = {0};
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256 * 4 * 3;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256 * 4 * kWalkWavesPerSimd;
     if (!slots[dev]) {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             cus <= 0)
             cus = 256;
-        slots[dev] = cus * 4 * 3;
+        slots[dev] = cus * 4 * kWalkWavesPerSimd;
     }
     return slots[dev];
 }
