@@ -96,32 +96,34 @@ __global__ void __launch_bounds__(256) coherence_batch_kernel(const CohItem* ite
 // 256 columns x CW_ROWS rows, the workgroup 256 columns x 4 CW_ROWS rows.
 using CohWalk3x10 = CohWalk<10, 3, 4, 1>;
 
+// Workgroup (bx, by): band bx (CW_BAND columns), output rows by*rows ..
+// by*rows + rows - 1.
 __device__ __forceinline__ void coherence_walk_block(const tops_ci16* M, const tops_cf32* S,
                                                      int w, int h, float* coh, int bx, int by,
-                                                     int vec, int rpw)
+                                                     int vec, int rows, CohWalk3x10::Lds& sm)
 {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int c0 = CW_COLS * (bx * 64 + lane);
-    const int r0 = (by * 4 + wv) * rpw;
-    if (c0 >= w || r0 >= h) return;
-    CohWalk3x10::walk(M, S, w, h, coh, c0, r0, min(r0 + rpw, h), vec != 0);
+    const int cb = bx * CW_BAND;
+    const int r0 = by * rows;
+    if (cb >= w || r0 >= h) return;   // uniform over the workgroup
+    CohWalk3x10::walk(M, S, w, h, coh, cb, r0, min(r0 + rows, h), vec != 0, sm);
 }
 
-// rpw: output rows per wave walk
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(CW_THREADS)
 coherence_walk_kernel(const tops_ci16* M, const tops_cf32* S, int w, int h, float* coh, int vec,
-                      int rpw)
+                      int rows)
 {
-    coherence_walk_block(M, S, w, h, coh, blockIdx.x, blockIdx.y, vec, rpw);
+    __shared__ CohWalk3x10::Lds sm;
+    coherence_walk_block(M, S, w, h, coh, blockIdx.x, blockIdx.y, vec, rows, sm);
 }
 
-// Batched form: work[i] = {item, block x, block y, vec}, CW_ROWS rows per wave.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
-coherence_walk_batch_kernel(const CohItem* items, const int4* work)
+// Batched form: work[i] = {item, band, row block, vec}, CW_ROWS rows per block.
+__global__ void __launch_bounds__(CW_THREADS) coherence_walk_batch_kernel(const CohItem* items,
+                                                                          const int4* work)
 {
+    __shared__ CohWalk3x10::Lds sm;
     const int4 wk = work[blockIdx.x];
     const CohItem& it = items[wk.x];
-    coherence_walk_block(it.m, it.s, it.w, it.h, it.coh, wk.y, wk.z, wk.w, CW_ROWS);
+    coherence_walk_block(it.m, it.s, it.w, it.h, it.coh, wk.y, wk.z, wk.w, CW_ROWS, sm);
 }
 
 bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az)
@@ -129,21 +131,23 @@ bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az)
     return win_rg == 10 && win_az == 3 && dx == 4 && dy == 1;
 }
 
+// pair loads: w even, master 8-byte, slave 16-byte, coherence 8-byte aligned
 int coherence_walk_vec(const void* m, const void* s, const void* coh, int w)
 {
-    return (w % 4 == 0) && ((uintptr_t)m % 16 == 0) && ((uintptr_t)s % 16 == 0) &&
-           ((uintptr_t)coh % 16 == 0);
+    return (w % 2 == 0) && ((uintptr_t)m % 8 == 0) && ((uintptr_t)s % 16 == 0) &&
+           ((uintptr_t)coh % 8 == 0);
 }
 
 int coherence_walk_count(int w, int h, int* nx, int* ny)
 {
-    *nx = (w + 64 * CW_COLS - 1) / (64 * CW_COLS);
-    *ny = (h + 4 * CW_ROWS - 1) / (4 * CW_ROWS);
+    *nx = (w + CW_BAND - 1) / CW_BAND;
+    *ny = (h + CW_ROWS - 1) / CW_ROWS;
     return *nx * *ny;
 }
 
-// Rows per wave for a single burst such that its waves fit in one round of
-// the GPU's slots (CUs x 4 SIMDs x 3 waves); at least CW_ROWS / 2.
+// Rows per workgroup for a single burst: 7 workgroups per CU in one round
+// (70 VGPRs allow 7 waves per SIMD, the LDS row buffers 9 workgroups per
+// CU), at least 16 rows.
 static int coherence_walk_rows(int w, int h)
 {
     static int slots = 0;
@@ -153,12 +157,11 @@ static int coherence_walk_rows(int w, int h)
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             cus <= 0)
             cus = 256;
-        slots = cus * 4 * 3;
+        slots = cus * 7;
     }
-    const int nx = (w + 64 * CW_COLS - 1) / (64 * CW_COLS);
-    const int ny = std::max(1, slots / 4 / nx);
-    const int rows_per_block = (h + ny - 1) / ny;
-    return std::max(CW_ROWS / 2, (rows_per_block + 3) / 4);
+    const int nx = (w + CW_BAND - 1) / CW_BAND;
+    const int ny = std::max(1, slots / nx);
+    return std::max(16, (h + ny - 1) / ny);
 }
 
 int coherence_tile_count(int w, int h, int* nx, int* ny)
@@ -173,8 +176,8 @@ int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_wor
 {
     if (n_work <= 0) return TOPS_OK;
     if (coherence_walk_fits(dx, dy, win_rg, win_az)) {
-        hipLaunchKernelGGL(coherence_walk_batch_kernel, dim3(n_work), dim3(256), 0, st, d_items,
-                           d_work);
+        hipLaunchKernelGGL(coherence_walk_batch_kernel, dim3(n_work), dim3(CW_THREADS), 0, st,
+                           d_items, d_work);
         TOPS_HIP(hipGetLastError());
         return TOPS_OK;
     }
@@ -196,11 +199,11 @@ int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h, int d
         hipLaunchKernelGGL(coherence_fp64_kernel, g, dim3(256), 0, st, m, s, w, h, dx, dy,
                            win_rg, win_az, coh);
     } else if (coherence_walk_fits(dx, dy, win_rg, win_az)) {
-        const int rpw = coherence_walk_rows(w, h);
-        const int nx = (w + 64 * CW_COLS - 1) / (64 * CW_COLS);
-        const int ny = (h + 4 * rpw - 1) / (4 * rpw);
-        hipLaunchKernelGGL(coherence_walk_kernel, dim3(nx, ny), dim3(256), 0, st, m, s, w, h, coh,
-                           coherence_walk_vec(m, s, coh, w), rpw);
+        const int rows = coherence_walk_rows(w, h);
+        const int nx = (w + CW_BAND - 1) / CW_BAND;
+        const int ny = (h + rows - 1) / rows;
+        hipLaunchKernelGGL(coherence_walk_kernel, dim3(nx, ny), dim3(CW_THREADS), 0, st, m, s, w,
+                           h, coh, coherence_walk_vec(m, s, coh, w), rows);
     } else {
         dim3 g((w + TW_COLS - 1) / TW_COLS, (h + TW_ROWS - 1) / TW_ROWS);
         hipLaunchKernelGGL(coherence_fast_kernel, g, dim3(256),

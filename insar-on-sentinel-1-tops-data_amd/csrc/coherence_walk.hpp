@@ -1,5 +1,6 @@
-// coherence_walk.hpp -- boxcar coherence with a compile-time window, one
-// thread per 4 adjacent output columns walking down a run of output rows.
+// coherence_walk.hpp -- boxcar coherence with a compile-time window: a
+// 256-thread workgroup walks down a band of 512 output columns, 2 adjacent
+// columns per thread.
 //
 // Same semantics as the LDS-tiled form (EstCoh.cu Cohbasic, 413-456, with
 // the zero padding of CohMatSet, 101-139): output (r, c) sums the products
@@ -7,13 +8,15 @@
 // c-DX .. c-DX+WR-1 (zero outside the burst) and is 0 where |S(r, c)| <
 // 0.001.
 //
-// Per input row the thread loads the WR+3 samples its 4 outputs need with
-// 16-byte vector loads (CInt16 master 4 per load, CF32 slave 2 per load),
-// forms their products in registers and the 4 range sums.  The sums share
-// their middle terms but are still plain sums of WR products each (no
-// running subtraction, so the rounding error stays relative to each
-// window's own sum of magnitudes).  The WA most recent rows of range sums
-// sit in a rolling register buffer; an output row is their direct sum.
+// Per input row each thread loads its own 2 samples (one 8-byte CInt16 pair,
+// one 16-byte CF32 pair; threads 0 .. WR-2 also load one halo sample of the
+// band), writes their products to an LDS row (double-buffered: one barrier
+// per row) and reads back the WR+1 products its two outputs need.  The two
+// range sums share their WR-1 middle terms but each is still a plain sum of
+// WR products (no running subtraction, so the rounding error stays relative
+// to each window's own sum of magnitudes).  The WA latest rows of range sums
+// sit in a rolling register buffer; an output row is their direct sum.  The
+// next row's samples are requested before the current row is summed.
 #pragma once
 
 #include "tops_internal.hpp"
@@ -21,8 +24,10 @@
 
 namespace tops {
 
-constexpr int CW_COLS = 4;    // output columns per thread
-constexpr int CW_ROWS = 32;   // output rows per thread walk
+constexpr int CW_THREADS = 256;                       // threads per workgroup
+constexpr int CW_PER_THREAD = 2;                      // output columns per thread
+constexpr int CW_BAND = CW_THREADS * CW_PER_THREAD;   // columns per workgroup
+constexpr int CW_ROWS = 32;                           // output rows per workgroup (batched form)
 
 __device__ __forceinline__ float4 cw_add(float4 a, float4 b)
 {
@@ -38,139 +43,125 @@ __device__ __forceinline__ float4 cw_prod(int mraw, float sr, float si)
 
 template <int WR, int WA, int DX, int DY>
 struct CohWalk {
-    static_assert(WR >= 4, "the shared-middle range sums need WR >= 4");
-    static constexpr int N = CW_COLS + WR - 1;          // samples a row needs
-    static constexpr int RDX = (DX + 3) / 4 * 4;        // DX rounded up to 4
-    static constexpr int OFF = RDX - DX;                // first needed sample in the load
-    static constexpr int L = (OFF + N + 3) / 4 * 4;     // samples loaded per row
+    static_assert(WR >= 2 && DX >= 0 && DX < WR && DY >= 0 && DY < WA, "window");
+    static constexpr int HALO_L = DX;               // columns left of the band
+    static constexpr int HALO_R = WR - 1 - DX;      // columns right of the band
+    static constexpr int ROW = CW_BAND + WR - 1;    // products per LDS row
+    static_assert(HALO_L + HALO_R <= CW_THREADS, "halo loads");
 
-    // raw samples of one input row: L master (packed CInt16) and L slave
-    // (CF32) samples from column c0 - RDX on
-    struct Raw {
-        int4 m[L / 4];
-        float4 s[L / 2];
+    struct Lds {
+        float4 row[2][ROW];
     };
 
-    // the row's raw samples (zero outside the burst).  vec: w % 4 == 0 and
-    // M, S, coh 16-byte aligned; rows in the middle of the burst then use
-    // 16-byte loads only.
+    // this thread's two samples of one row (zero outside the burst), plus
+    // its halo sample when it has one
+    struct Raw {
+        int m0, m1, mh;
+        float4 s;        // (re0, im0, re1, im1)
+        float2 sh;
+    };
+
+    // vec: w even and M, S, coh 8- / 16- / 8-byte aligned
     __device__ static void fetch(const tops_ci16* M, const tops_cf32* S, int w, int h, int gr,
-                                 int c0, bool vec, Raw& raw)
+                                 int cb, bool vec, Raw& raw)
     {
-        if (gr < 0 || gr >= h) {
-#pragma unroll
-            for (int q = 0; q < L / 4; q++) raw.m[q] = make_int4(0, 0, 0, 0);
-#pragma unroll
-            for (int q = 0; q < L / 2; q++) raw.s[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-            return;
-        }
+        const int t = threadIdx.x;
+        const int c = cb + CW_PER_THREAD * t;
+        raw.m0 = raw.m1 = raw.mh = 0;
+        raw.s = make_float4(0.f, 0.f, 0.f, 0.f);
+        raw.sh = make_float2(0.f, 0.f);
+        if (gr < 0 || gr >= h) return;
         const size_t rb = (size_t)gr * w;
-        const int lo = c0 - RDX;
-        if (vec && lo >= 0 && lo + L <= w) {
-            // aligned: rb + lo is a multiple of 4 samples (w % 4 == 0, c0 % 4 == 0)
-            const int4* mq = reinterpret_cast<const int4*>(M + rb + lo);
-            const float4* sq = reinterpret_cast<const float4*>(S + rb + lo);
-#pragma unroll
-            for (int q = 0; q < L / 4; q++) raw.m[q] = mq[q];
-#pragma unroll
-            for (int q = 0; q < L / 2; q++) raw.s[q] = sq[q];
+        const int* mi = reinterpret_cast<const int*>(M);
+        if (vec && c + 1 < w) {
+            // rb + c is even: the pair loads are aligned
+            const int2 m2 = *reinterpret_cast<const int2*>(mi + rb + c);
+            raw.m0 = m2.x;
+            raw.m1 = m2.y;
+            raw.s = *reinterpret_cast<const float4*>(S + rb + c);
         } else {
-            const int* mi = reinterpret_cast<const int*>(M);
-            int m[L];
-            float sv[2 * L];
-#pragma unroll
-            for (int k = 0; k < L; k++) {
-                const int c = lo + k;
-                m[k] = 0;
-                sv[2 * k] = sv[2 * k + 1] = 0.f;
-                if (c >= 0 && c < w) {
-                    const tops_cf32 t = S[rb + c];
-                    m[k] = mi[rb + c];
-                    sv[2 * k] = t.x;
-                    sv[2 * k + 1] = t.y;
-                }
+            if (c < w) {
+                raw.m0 = mi[rb + c];
+                raw.s.x = S[rb + c].x;
+                raw.s.y = S[rb + c].y;
             }
-#pragma unroll
-            for (int q = 0; q < L / 4; q++)
-                raw.m[q] = make_int4(m[4 * q], m[4 * q + 1], m[4 * q + 2], m[4 * q + 3]);
-#pragma unroll
-            for (int q = 0; q < L / 2; q++)
-                raw.s[q] = make_float4(sv[4 * q], sv[4 * q + 1], sv[4 * q + 2], sv[4 * q + 3]);
+            if (c + 1 < w) {
+                raw.m1 = mi[rb + c + 1];
+                raw.s.z = S[rb + c + 1].x;
+                raw.s.w = S[rb + c + 1].y;
+            }
+        }
+        if (t < HALO_L + HALO_R) {
+            const int ch = t < HALO_L ? cb - HALO_L + t : cb + CW_BAND + (t - HALO_L);
+            if (ch >= 0 && ch < w) {
+                raw.mh = mi[rb + ch];
+                raw.sh = make_float2(S[rb + ch].x, S[rb + ch].y);
+            }
         }
     }
 
-    // products of the N samples the 4 outputs need (columns c0-DX .. c0-DX+N-1)
-    __device__ static void products(const Raw& raw, float4 (&v)[N])
+    // products of a fetched row into LDS row buffer b
+    __device__ static void store(const Raw& raw, Lds& sm, int b)
     {
-#pragma unroll
-        for (int k = 0; k < N; k++) {
-            const int e = OFF + k;
-            const int4 mq = raw.m[e / 4];
-            const int mv = (e % 4 == 0) ? mq.x : (e % 4 == 1) ? mq.y : (e % 4 == 2) ? mq.z : mq.w;
-            const float4 sq = raw.s[e / 2];
-            const float sr = (e % 2 == 0) ? sq.x : sq.z, si = (e % 2 == 0) ? sq.y : sq.w;
-            v[k] = cw_prod(mv, sr, si);
+        const int t = threadIdx.x;
+        sm.row[b][HALO_L + CW_PER_THREAD * t] = cw_prod(raw.m0, raw.s.x, raw.s.y);
+        sm.row[b][HALO_L + CW_PER_THREAD * t + 1] = cw_prod(raw.m1, raw.s.z, raw.s.w);
+        if (t < HALO_L + HALO_R) {
+            const int k = t < HALO_L ? t : HALO_L + CW_BAND + (t - HALO_L);
+            sm.row[b][k] = cw_prod(raw.mh, raw.sh.x, raw.sh.y);
         }
     }
 
-    // range sums of the 4 outputs: s_j = sum of v[j .. j+WR-1]
-    __device__ static void range_sums(const float4 (&v)[N], float4 (&s)[CW_COLS])
-    {
-        float4 mid = v[3];
-#pragma unroll
-        for (int k = 4; k < WR; k++) mid = cw_add(mid, v[k]);
-        const float4 a12 = cw_add(v[1], v[2]);
-        const float4 c01 = cw_add(v[WR], v[WR + 1]);
-        s[0] = cw_add(cw_add(v[0], a12), mid);
-        s[1] = cw_add(cw_add(a12, mid), v[WR]);
-        s[2] = cw_add(cw_add(v[2], mid), c01);
-        s[3] = cw_add(mid, cw_add(c01, v[WR + 2]));
-    }
-
-    // output rows r_begin .. r_end-1, columns c0 .. c0+3
+    // output rows r_begin .. r_end-1 (the same for every thread of the
+    // workgroup) of the band starting at column cb
     __device__ static void walk(const tops_ci16* M, const tops_cf32* S, int w, int h, float* coh,
-                                int c0, int r_begin, int r_end, bool vec)
+                                int cb, int r_begin, int r_end, bool vec, Lds& sm)
     {
-        float4 H[WA][CW_COLS];     // range sums of the WA latest input rows, by slot
-        unsigned ok[WA];           // |S| >= 0.001 flags of those rows' own 4 columns
+        const int t = threadIdx.x;
+        const int c = cb + CW_PER_THREAD * t;      // this thread's first column
+        float4 H[WA][CW_PER_THREAD];               // range sums of the WA latest rows, by slot
+        unsigned ok[WA];                           // |S| >= 0.001 flags of those rows' own columns
         const int first = r_begin - DY;            // first input row
         const int last = r_end - 1 - DY + WA - 1;  // last input row
-        Raw nxt;                                   // next row, loads in flight
-        fetch(M, S, w, h, first, c0, vec, nxt);
+        Raw nxt;
+        fetch(M, S, w, h, first, cb, vec, nxt);
         for (int base = first; base <= last; base += WA) {
 #pragma unroll
             for (int q = 0; q < WA; q++) {
                 const int gr = base + q;
-                if (gr > last) break;
+                if (gr > last) break;               // uniform over the workgroup
+                const int b = (gr - first) & 1;
                 const Raw cur = nxt;
-                if (gr < last) fetch(M, S, w, h, gr + 1, c0, vec, nxt);
-                float4 v[N];
-                products(cur, v);
-                range_sums(v, H[q]);
-                unsigned f = 0;
+                if (gr < last) fetch(M, S, w, h, gr + 1, cb, vec, nxt);
+                store(cur, sm, b);
+                ok[q] = (amplitude_ok(cur.s.x * cur.s.x + cur.s.y * cur.s.y) ? 1u : 0u) |
+                        (amplitude_ok(cur.s.z * cur.s.z + cur.s.w * cur.s.w) ? 2u : 0u);
+                __syncthreads();
+                // window of output c: LDS entries 2t .. 2t+WR-1; of c+1: one more
+                const float4* rp = &sm.row[b][CW_PER_THREAD * t];
+                float4 mid = rp[1];
 #pragma unroll
-                for (int j = 0; j < CW_COLS; j++)
-                    f |= (amplitude_ok(v[DX + j].z) ? 1u : 0u) << j;
-                ok[q] = f;
+                for (int k = 2; k < WR; k++) mid = cw_add(mid, rp[k]);
+                H[q][0] = cw_add(rp[0], mid);
+                H[q][1] = cw_add(mid, rp[WR]);
                 if (gr - first < WA - 1) continue;
                 // output row r: input rows r-DY .. r-DY+WA-1 are in the buffer
                 const int r = gr - (WA - 1) + DY;
                 const int slot_r = ((q - (WA - 1) + DY) % WA + WA) % WA;
-                float out[CW_COLS];
+                float out[CW_PER_THREAD];
 #pragma unroll
-                for (int j = 0; j < CW_COLS; j++) {
+                for (int j = 0; j < CW_PER_THREAD; j++) {
                     float4 a = H[0][j];
 #pragma unroll
-                    for (int t = 1; t < WA; t++) a = cw_add(a, H[t][j]);
+                    for (int u = 1; u < WA; u++) a = cw_add(a, H[u][j]);
                     out[j] = (ok[slot_r] >> j) & 1u ? coherence_of(a.x, a.y, a.z, a.w) : 0.0f;
                 }
-                float* dst = coh + (size_t)r * w + c0;
-                if (vec && c0 + CW_COLS <= w) {
-                    *reinterpret_cast<float4*>(dst) = make_float4(out[0], out[1], out[2], out[3]);
+                float* dst = coh + (size_t)r * w + c;
+                if (vec && c + 1 < w) {
+                    *reinterpret_cast<float2*>(dst) = make_float2(out[0], out[1]);
                 } else {
-#pragma unroll
-                    for (int j = 0; j < CW_COLS; j++)
-                        if (c0 + j < w) dst[j] = out[j];
+                    if (c < w) dst[0] = out[0];
+                    if (c + 1 < w) dst[1] = out[1];
                 }
             }
         }

@@ -118,7 +118,7 @@ int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_wor
                            int dy, int win_rg, int win_az, hipStream_t st);
 int coherence_tile_count(int w, int h, int* nx, int* ny);
 // walking form (3 az x 10 rg window, dx = 4, dy = 1); batched work items are
-// {item, block x, block y, coherence_walk_vec(...)}
+// {item, column band, row block, coherence_walk_vec(...)}
 bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az);
 int coherence_walk_vec(const void* m, const void* s, const void* coh, int w);
 int coherence_walk_count(int w, int h, int* nx, int* ny);
