@@ -411,10 +411,13 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
         int iy = 0, fy = 0, ix = 0, fx = 0, reg = -1;
         {
 #pragma clang fp contract(off)
-            const double* t = sm.rowt[rr];
-            const double pn = sm.pos[0][lane];
-            double ys = t[0] + sm.pos[1][lane] + t[1] + t[2] * pn + sm.pos[2][lane] + p.az_shift;
-            double xs = t[3] + sm.pos[3][lane] + t[4] + t[5] * pn + sm.pos[4][lane];
+            // all LDS operands first (their reads overlap), then the sums in order
+            const double2* t2 = reinterpret_cast<const double2*>(sm.rowt[rr]);
+            const double2 ta = t2[0], tb = t2[1], tc = t2[2];
+            const double pn = sm.pos[0][lane], q1 = sm.pos[1][lane], q2 = sm.pos[2][lane];
+            const double q3 = sm.pos[3][lane], q4 = sm.pos[4][lane];
+            double ys = ta.x + q1 + ta.y + tb.x * pn + q2 + p.az_shift;
+            double xs = tb.y + q3 + tc.x + tc.y * pn + q4;
             ys = fmin(fmax(ys, -1.0e9), 1.0e9);
             xs = fmin(fmax(xs, -1.0e9), 1.0e9);
             iy = (int)trunc(ys);
