@@ -259,6 +259,21 @@ def run_iw_pair(args, tops, torch):
                                      coh.data_ptr(), N.MODE_FP64, sp))
     stream.synchronize()
     fp64_s = (time.perf_counter() - t0) / nfp
+    # the same bursts as per-burst fast launches (no plan, no graph)
+    def per_burst():
+        for (d, nl, width, *_r), g, (out, coh) in zip(specs, geoms, outs):
+            N.check(L.tops_resample_burst(ctx, C.byref(g), d["S"].data_ptr(), d["kt"].data_ptr(),
+                                          d["tref"].data_ptr(), d["fdc"].data_ptr(),
+                                          out.data_ptr(), N.MODE_FAST, sp))
+            N.check(L.tops_coherence(d["M"].data_ptr(), out.data_ptr(), width, nl, 4, 1, 10, 3,
+                                     coh.data_ptr(), N.MODE_FAST, sp))
+    per_burst()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        per_burst()
+    stream.synchronize()
+    per_burst_s = (time.perf_counter() - t0) / args.steps
     L.tops_plan_destroy(plan)
     L.tops_ctx_destroy(ctx)
     value = samples / step_s / 1e6
@@ -272,6 +287,7 @@ def run_iw_pair(args, tops, torch):
                    "parallelism": "single GPU"},
         "vs_plain_fp64": round(fp64_s / step_s, 3),
         "plain_fp64_msamples_per_s": round(samples / fp64_s / 1e6, 2),
+        "per_burst_launches_msamples_per_s": round(samples / per_burst_s / 1e6, 2),
         "path_hbm_frac": round(PATH_BYTES * samples / step_s / 1e9 / HBM_PEAK_GBS, 4)}))
 
 

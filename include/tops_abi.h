@@ -198,18 +198,22 @@ typedef struct {
 } tops_burst_item;
 
 typedef struct tops_plan tops_plan;
-/* One plan = one resample launch over every burst's strips plus one
- * coherence launch over every burst's tiles (window coh_win_az x
- * coh_win_rg starting (win-1)/2 before the sample).  strip_rows <= 0 picks
- * the default strip height. */
+/* A plan resamples every burst and forms its coherence (window coh_win_az
+ * x coh_win_rg starting (win-1)/2 before the sample) on one stream, on one
+ * of two schedules: strip_rows <= 0, one resample and one coherence launch
+ * per burst with heights sized to each burst; strip_rows > 0, one batched
+ * resample launch over every burst's strips of that height, then one
+ * batched coherence launch. */
 tops_plan* tops_plan_create(tops_ctx* ctx, const tops_burst_item* items, int n_items,
                             int coh_win_az, int coh_win_rg, int strip_rows);
 int tops_plan_run(tops_plan* plan, void* stream);
 /* Capture the plan's launches in a hipGraph on a non-default stream; later
  * tops_plan_run calls replay it. */
 int tops_plan_capture_graph(tops_plan* plan, void* stream);
-/* Time the candidate strip heights on this plan and keep the fastest
- * (cached in the file named by TOPS_TUNE_CACHE when set). */
+/* Time the candidate schedules on this plan (per-burst, and batched at
+ * several strip heights) and keep the fastest; *best_strip_rows gets its
+ * strip_rows value (0 = per-burst).  Cached in the file named by
+ * TOPS_TUNE_CACHE when set. */
 int tops_plan_tune(tops_plan* plan, void* stream, int* best_strip_rows);
 int tops_plan_strip_rows(const tops_plan* plan);
 void tops_plan_destroy(tops_plan* plan);

@@ -19,7 +19,13 @@ struct tops_plan {
     tops_ctx* ctx = nullptr;
     std::vector<tops_burst_item> items;
     int coh_win_az = 3, coh_win_rg = 10;
-    int strip_rows = kDefaultStripRows;
+    // schedule: 0 = one launch pair per burst (resample, then coherence) with
+    // each burst's automatic heights; > 0 = one batched launch of all bursts'
+    // strips of this height, then one batched coherence launch
+    int strip_rows = 0;
+    std::vector<ResampleParams> h_params;
+    std::vector<CohItem> h_coh;
+    std::vector<int> coh_of;   // index into h_coh per item, or -1
     ResampleParams* d_params = nullptr;
     int4* d_work_rs = nullptr;
     int n_work_rs = 0;
@@ -62,14 +68,17 @@ int fill_params(tops_plan* p)
         r.krg64 = p->ctx->d_krg64;
         r.out = it.out;
     }
+    p->h_params = prm;
     TOPS_HIP(hipMalloc(&p->d_params, sizeof(ResampleParams) * prm.size()));
     TOPS_HIP(hipMemcpy(p->d_params, prm.data(), sizeof(ResampleParams) * prm.size(),
                        hipMemcpyHostToDevice));
     std::vector<CohItem> ci;
     std::vector<int4> wc;
+    p->coh_of.assign(p->items.size(), -1);
     for (size_t i = 0; i < p->items.size(); i++) {
         const tops_burst_item& it = p->items[i];
         if (!it.master || !it.coh) continue;
+        p->coh_of[i] = (int)ci.size();
         CohItem c;
         c.m = it.master;
         c.s = it.out;
@@ -88,6 +97,7 @@ int fill_params(tops_plan* p)
             for (int x = 0; x < nx; x++) wc.push_back(make_int4((int)ci.size(), x, y, tag));
         ci.push_back(c);
     }
+    p->h_coh = ci;
     p->n_work_coh = (int)wc.size();
     if (!ci.empty()) {
         TOPS_HIP(hipMalloc(&p->d_coh, sizeof(CohItem) * ci.size()));
@@ -102,6 +112,7 @@ int fill_params(tops_plan* p)
 // interleaved row-band by row-band so uneven bursts spread over the GPU
 int fill_work(tops_plan* p)
 {
+    if (p->strip_rows <= 0) return TOPS_OK;   // per-burst launches need no table
     std::vector<int4> w;
     int max_ny = 0;
     std::vector<int> nxs(p->items.size()), nys(p->items.size());
@@ -127,10 +138,23 @@ int fill_work(tops_plan* p)
 
 int enqueue(tops_plan* p, hipStream_t s)
 {
+    const int dx = (p->coh_win_rg - 1) / 2, dy = (p->coh_win_az - 1) / 2;
+    if (p->strip_rows <= 0) {
+        for (size_t i = 0; i < p->h_params.size(); i++) {
+            int rc = launch_resample_fast(p->h_params[i], s, 0);
+            if (rc) return rc;
+            if (p->coh_of[i] >= 0) {
+                const CohItem& c = p->h_coh[p->coh_of[i]];
+                rc = launch_coherence(c.m, c.s, c.w, c.h, dx, dy, p->coh_win_rg, p->coh_win_az,
+                                      c.coh, TOPS_MODE_FAST, s);
+                if (rc) return rc;
+            }
+        }
+        return TOPS_OK;
+    }
     int rc = launch_resample_batch(p->d_params, p->d_work_rs, p->n_work_rs, s);
     if (rc) return rc;
-    return launch_coherence_batch(p->d_coh, p->d_work_coh, p->n_work_coh,
-                                  (p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2, p->coh_win_rg,
+    return launch_coherence_batch(p->d_coh, p->d_work_coh, p->n_work_coh, dx, dy, p->coh_win_rg,
                                   p->coh_win_az, s);
 }
 
@@ -169,7 +193,7 @@ tops_plan* tops_plan_create(tops_ctx* ctx, const tops_burst_item* items, int n_i
     p->items.assign(items, items + n_items);
     p->coh_win_az = coh_win_az;
     p->coh_win_rg = coh_win_rg;
-    p->strip_rows = strip_rows > 0 ? strip_rows : kDefaultStripRows;
+    p->strip_rows = strip_rows > 0 ? strip_rows : 0;
     if (fill_params(p) || fill_work(p)) {
         tops_plan_destroy(p);
         return nullptr;
@@ -221,9 +245,10 @@ int tops_plan_capture_graph(tops_plan* p, void* stream)
 
 int tops_plan_strip_rows(const tops_plan* p) { return p ? p->strip_rows : 0; }
 
-// Autotuner: times every candidate strip height on the plan's bursts
-// (warm-up + 3 timed runs each, hipEvents on `stream`) and keeps the
-// fastest.  With TOPS_TUNE_CACHE set, the choice is read from / appended to
+// Autotuner: times every candidate schedule (per-burst launches, or one
+// batched launch at each candidate strip height) on the plan's bursts,
+// resample and coherence together (warm-up + 3 timed runs each, hipEvents
+// on `stream`), and keeps the fastest.  With TOPS_TUNE_CACHE set, the choice is read from / appended to
 // that text file ("<key> <strip_rows>" lines, key = the burst shapes).
 int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
 {
@@ -254,9 +279,9 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
             fclose(f);
         }
     }
-    // fixed heights, plus the heights that fill 1, 2 and 3 rounds of the
-    // GPU's walker wave slots with no partial round
-    std::vector<int> cand = {32, 64, 128, 256, 512};
+    // per-burst launches (0), fixed batched heights, and the batched heights
+    // that fill 1, 2 and 3 rounds of the GPU's walker wave slots
+    std::vector<int> cand = {0, 32, 64, 128, 256, 512};
     {
         std::vector<int> ls, ps;
         for (auto& it : p->items) {
@@ -278,11 +303,11 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
     for (int c : cand) {
         p->strip_rows = c;
         if ((rc = fill_work(p))) break;
-        if ((rc = launch_resample_batch(p->d_params, p->d_work_rs, p->n_work_rs, s))) break;
+        if ((rc = enqueue(p, s))) break;   // warm-up
         float tot = 0.f;
         for (int k = 0; k < 3 && !rc; k++) {
             hipError_t e = hipEventRecord(e0, s);
-            if (e == hipSuccess) rc = launch_resample_batch(p->d_params, p->d_work_rs, p->n_work_rs, s);
+            if (e == hipSuccess) rc = enqueue(p, s);
             if (e == hipSuccess) e = hipEventRecord(e1, s);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.f;

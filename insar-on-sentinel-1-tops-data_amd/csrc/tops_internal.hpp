@@ -102,7 +102,6 @@ int launch_resample_batch(const ResampleParams* d_items, const int4* d_work, int
 int resample_strip_count(int m_lines, int m_pixels, int strip_rows, int* nx, int* ny);
 int resample_wave_slots();
 int resample_rows_for_rounds(const int* lines, const int* pixels, int n, int rounds);
-constexpr int kDefaultStripRows = 128;
 
 // launchers (coherence.hip)
 int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h,

@@ -1,4 +1,4 @@
-"""Batched plans (many bursts in one launch, hipGraph replay, strip-height
+"""Plans over many bursts (batched or per-burst schedule, hipGraph replay, strip-height
 tuning) against the single-burst path and the FP64 oracle."""
 import ctypes as C
 
@@ -93,6 +93,23 @@ def test_plan_matches_single_and_oracle(tops, orc, dev, batch):
     L.tops_plan_destroy(plan)
 
 
+def test_plan_per_burst_schedule_is_the_single_burst_path(tops, dev, batch):
+    """strip_rows 0: the plan issues the single-burst launches, so its output
+    is bit-identical to tops_resample_burst + tops_coherence."""
+    N = tops.native
+    L = N.load()
+    plan = L.tops_plan_create(batch["ctx"], batch["items"], 5, 3, 10, 0)
+    assert plan, N.last_error()
+    assert L.tops_plan_strip_rows(plan) == 0
+    N.check(L.tops_plan_run(plan, None), "tops_plan_run")
+    for i, (b, d) in enumerate(zip(batch["bursts"], batch["dev"])):
+        got = dev.get(d["out"], np.complex64, (b["L"], b["P"])).copy()
+        coh = dev.get(d["coh"], np.float32, (b["L"], b["P"])).copy()
+        s_out, s_coh = _single(tops, dev, batch["ctx"], b, d)
+        assert np.array_equal(got, s_out) and np.array_equal(coh, s_coh), i
+    L.tops_plan_destroy(plan)
+
+
 def test_plan_tune_and_graph_replay(tops, dev, batch):
     N = tops.native
     L = N.load()
@@ -104,8 +121,9 @@ def test_plan_tune_and_graph_replay(tops, dev, batch):
     best = C.c_int(0)
     print("  plan: tuning", flush=True)
     N.check(L.tops_plan_tune(plan, sp, C.byref(best)), "tops_plan_tune")
-    # fixed candidates or a height that fills whole rounds of wave slots
-    assert 16 <= best.value <= 4096
+    # per-burst launches (0), a fixed height, or a height that fills whole
+    # rounds of wave slots
+    assert best.value == 0 or 16 <= best.value <= 4096
     assert L.tops_plan_strip_rows(plan) == best.value
     N.check(L.tops_plan_run(plan, sp))
     s.synchronize()
