@@ -291,6 +291,68 @@ def run_iw_pair(args, tops, torch):
         "path_hbm_frac": round(PATH_BYTES * samples / step_s / 1e9 / HBM_PEAK_GBS, 4)}))
 
 
+def run_subswath(args, tops, torch):
+    """The host stage layer on one subswath of 9 bursts of 1500 x 21000 with
+    ~100-line overlaps: GeometricCoreg, DerampAndResample and ESDAndCoh
+    (Main.cpp:107-121) over host-resident SLCs, as the reference's stages
+    run them (host buffers in and out, so PCIe transfers are inside the
+    timing).  value = master-grid samples of the subswath / the three
+    stages' wall time."""
+    import tops_amd.pipeline as pl
+    t0 = time.perf_counter()
+    run = pl.SubswathRun(tops.synth.subswath_iw1())
+    print(f"[bench] subswath generated in {time.perf_counter() - t0:.0f} s", file=sys.stderr,
+          flush=True)
+    samples = run.nb * run.lpb * run.spb
+    torch.cuda.synchronize()
+    times = {"geometric_coreg": [], "deramp_and_resample": [], "esd_and_coh": []}
+    out = None
+    for k in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        ca, cr = run.geometric_coreg()
+        t1 = time.perf_counter()
+        res1 = run.deramp_and_resample(ca, cr)
+        t2 = time.perf_counter()
+        res2, coh, shifts = run.esd_and_coh(ca, cr, res1, coh_win=(3, 10))
+        t3 = time.perf_counter()
+        if k >= args.warmup:
+            times["geometric_coreg"].append(t1 - t0)
+            times["deramp_and_resample"].append(t2 - t1)
+            times["esd_and_coh"].append(t3 - t2)
+        out = shifts
+    ms = {k: round(float(np.median(v)) * 1e3, 2) for k, v in times.items()}
+    total_s = sum(ms.values()) / 1e3
+    # the same ESDAndCoh flow with the subswath resident in HBM
+    # (tops_subswath_esd_coh: pass 1, ESD in place, device shifts, pass 2,
+    # coherence), timed with events on its stream
+    ds = pl.DeviceSubswath(run, ca, cr, coh_win=(3, 10))
+    st = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        ds.enqueue(st)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(args.steps):
+        ds.enqueue(st)
+    e1.record(st)
+    torch.cuda.synchronize()
+    dev_ms = e0.elapsed_time(e1) / args.steps
+    dev_shifts = ds.results()[2]
+    ds.close()
+    print(json.dumps({
+        "metric": METRIC, "value": round(samples / total_s / 1e6, 2), "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total_s * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "subswath", "bursts": run.nb, "burst": [run.lpb, run.spb],
+                   "samples": samples, "coh_window": [3, 10], "esd_window": [6, 24],
+                   "host_buffers": True, "parallelism": "single GPU"},
+        "stage_ms": ms, "esd_shifts": [round(float(v), 6) for v in out],
+        "device_esd_and_coh": {"ms": round(dev_ms, 3),
+                               "msamples_per_s": round(samples / dev_ms / 1e3, 1),
+                               "esd_shifts": [round(float(v), 6) for v in dev_shifts]}}))
+
+
 def run_stack(args, tops, torch):
     """One master (3 IW2 bursts of 1500 x 25000) against 16 slaves, each with
     its own offset field (az 0-0.5 px, rg 0-3 px) and Doppler centroid
@@ -444,6 +506,8 @@ def main():
         return run_iw_pair(args, tops, torch)
     if args.workload == "stack":
         return run_stack(args, tops, torch)
+    if args.workload == "subswath":
+        return run_subswath(args, tops, torch)
     N = tops.native
     L = N.load()
     pair = workloads(tops, args.workload)

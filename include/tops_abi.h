@@ -22,6 +22,7 @@
 #ifndef TOPS_ABI_H
 #define TOPS_ABI_H
 
+#include <stddef.h>
 #include <stdint.h>
 #ifndef __cplusplus
 #include <stdbool.h>
@@ -217,6 +218,39 @@ int tops_plan_capture_graph(tops_plan* plan, void* stream);
 int tops_plan_tune(tops_plan* plan, void* stream, int* best_strip_rows);
 int tops_plan_strip_rows(const tops_plan* plan);
 void tops_plan_destroy(tops_plan* plan);
+
+/* ESDAndCoh (Func.cpp:2496-2763) over one subswath held in HBM, with no
+ * host round trip between its steps: pass-1 resample of every burst, ESD
+ * over the overlap bands read in place, shifts formed on the device
+ * (|shift| > 100 -> 0), pass-2 resample of bursts 1..n-1 with
+ * AzimuthShift = shift[b-1] (master lines per burst and azimuth interval,
+ * Func.cpp:2701-2705), coherence of every burst.  All launches go to
+ * `stream` in that order (capturable in one hipGraph); nothing is
+ * allocated.  Rasters are whole subswaths, burst after burst. */
+typedef struct tops_subswath_desc {
+    int n_bursts;
+    int m_lines_per_burst, m_samples_per_burst;
+    int s_lines_per_burst, s_samples_per_burst;
+    double m_az_time_interval, s_az_time_interval;
+    double spectral_sep;            /* GetSpectralSep of the master swath   */
+    const int* overlap_lines;       /* host [n_bursts-1], EstOverlapSize    */
+    int esd_win_az, esd_win_rg;     /* reference: 6 x 24                    */
+    float threshold;                /* reference: 0.3                       */
+    int coh_win_az, coh_win_rg;
+} tops_subswath_desc;
+/* Bytes of device workspace tops_subswath_esd_coh needs (0: bad desc). */
+size_t tops_subswath_workspace_size(const tops_subswath_desc* desc);
+int tops_subswath_esd_coh(tops_ctx* ctx, const tops_subswath_desc* desc,
+                          const tops_ci16* d_master,  /* [n*mLpb x mSpb]        */
+                          const tops_ci16* d_slave,   /* [n*sLpb x sSpb]        */
+                          const double* d_kt,         /* [n x sSpb] (+ tref, fdc) */
+                          const double* d_tref, const double* d_fdc,
+                          const double* cpm_az,       /* host [n x 6]           */
+                          const double* cpm_rg,
+                          tops_cf32* d_reslave,       /* out [n*mLpb x mSpb]    */
+                          float* d_coh,               /* out [n*mLpb x mSpb] or NULL */
+                          double* d_shifts,           /* out device [n-1]       */
+                          void* d_workspace, size_t workspace_bytes, void* stream);
 
 /* Geocoreg orbit description for one acquisition (device-side copy is made). */
 typedef struct {

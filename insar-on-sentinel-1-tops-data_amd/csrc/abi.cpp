@@ -226,6 +226,7 @@ int tops_resample_burst(tops_ctx* ctx, const tops_burst_geom* g, const tops_ci16
     p.mom = ctx->d_mom;
     p.kaz_sh = ctx->d_kaz_sh;
     p.krg64 = ctx->d_krg64;
+    p.az_shift_dev = nullptr;
     p.out = d_out;
     hipStream_t s = (hipStream_t)stream;
     return mode == TOPS_MODE_FAST ? launch_resample_fast(p, s) : launch_resample_fp64(ctx, p, s);

@@ -12,7 +12,8 @@
 //   dd           : (Mf conj Sf) conj(Mb conj Sb)              (DoubleDifference4OneD)
 //   phasor       : 0 if coh < thr or coh >= 1, else exp(j arg dd) (MaskOut)
 //   sum_i        : sum of phasors over the band               (reduce0 x3)
-// The shift itself, atan2(sum_i) / (2 pi df dt), is formed on the host.
+// The shift itself, atan2(sum_i) / (2 pi df dt), is formed on the host, or
+// on the device by esd_shift_kernel for the device-resident two-pass flow.
 //
 // One fused kernel per call: each 16x64 tile computes both band coherences
 // from LDS (tile_window_sums), the double difference and the masked
@@ -22,10 +23,13 @@
 #include "tops_internal.hpp"
 #include "tile_window.hpp"
 
+#include <vector>
+
 namespace tops {
 
 struct EsdBand {
-    int line_offset;   // first line of the band in the concatenated arrays
+    int f_offset;      // first line of the forward band in mf / sf
+    int b_offset;      // first line of the backward band in mb / sb
     int lines;         // n_i
 };
 
@@ -62,13 +66,14 @@ __global__ void __launch_bounds__(256) esd_band_kernel(EsdParams p)
         if (threadIdx.x == 0) *out = make_double2(0.0, 0.0);
         return;
     }
-    const size_t off = (size_t)p.band[b].line_offset * p.pixels;
+    const size_t off_f = (size_t)p.band[b].f_offset * p.pixels;
+    const size_t off_b = (size_t)p.band[b].b_offset * p.pixels;
     const int dh = (p.win_az - 1) / 2, dw = (p.win_rg - 1) / 2;
     WindowSums wf[4], wb[4];
     float2 cf[4], cb[4];
-    tile_window_sums(p.mf + off, p.sf + off, p.pixels, n, r0, c0, dw, dh, p.win_rg,
+    tile_window_sums(p.mf + off_f, p.sf + off_f, p.pixels, n, r0, c0, dw, dh, p.win_rg,
                      p.win_az, smem, wf, cf);
-    tile_window_sums(p.mb + off, p.sb + off, p.pixels, n, r0, c0, dw, dh, p.win_rg,
+    tile_window_sums(p.mb + off_b, p.sb + off_b, p.pixels, n, r0, c0, dw, dh, p.win_rg,
                      p.win_az, smem, wb, cb);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float sx = 0.f, sy = 0.f, ex = 0.f, ey = 0.f;
@@ -128,6 +133,26 @@ __global__ void __launch_bounds__(256) esd_final_kernel(const double2* partials,
     }
 }
 
+// shift_i = atan2(sum_i) / (2 pi df dt) (EstESD.cu:1213-1217), 0 when its
+// magnitude exceeds 100 lines (Func.cpp:2612-2621)
+__global__ void esd_shift_kernel(const double* sums, int n, double df, double dt, double* shifts)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sh = atan2(sums[2 * i + 1], sums[2 * i]) / (2 * kPi * df * dt);
+    shifts[i] = fabs(sh) > 100 ? 0.0 : sh;
+}
+
+int launch_esd_shifts(const double* d_sums, int n, double df, double dt, double* d_shifts,
+                      hipStream_t st)
+{
+    if (n <= 0) return TOPS_OK;
+    hipLaunchKernelGGL(esd_shift_kernel, dim3((n + 63) / 64), dim3(64), 0, st, d_sums, n, df, dt,
+                       d_shifts);
+    TOPS_HIP(hipGetLastError());
+    return TOPS_OK;
+}
+
 int esd_partial_capacity(const int* overlap_size, int n_overlap, int pixels)
 {
     int maxn = 0;
@@ -141,6 +166,18 @@ int launch_esd(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* sf,
                int win_az, int win_rg, float thr, double* d_sums, double* d_partials,
                int /*max_blocks_per_band*/, hipStream_t st)
 {
+    // bands concatenated in order in all four arrays
+    std::vector<int> off(n_overlap > 0 ? n_overlap : 1, 0);
+    for (int i = 1; i < n_overlap; i++) off[i] = off[i - 1] + overlap_size[i - 1];
+    return launch_esd_bands(mf, mb, sf, sb, off.data(), off.data(), overlap_size, n_overlap,
+                            pixels, win_az, win_rg, thr, d_sums, d_partials, st);
+}
+
+int launch_esd_bands(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* sf,
+                     const tops_cf32* sb, const int* f_offset, const int* b_offset,
+                     const int* overlap_size, int n_overlap, int pixels, int win_az, int win_rg,
+                     float thr, double* d_sums, double* d_partials, hipStream_t st)
+{
     if (n_overlap <= 0) return TOPS_OK;
     if (n_overlap > kMaxEsdBands)
         return set_error(TOPS_ERR_ARG, "EstESDShifts: at most %d overlaps per call", kMaxEsdBands);
@@ -150,11 +187,11 @@ int launch_esd(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* sf,
     EsdParams p;
     p.mf = mf; p.mb = mb; p.sf = sf; p.sb = sb;
     p.pixels = pixels; p.win_az = win_az; p.win_rg = win_rg; p.thr = thr;
-    int maxn = 0, off = 0;
+    int maxn = 0;
     for (int i = 0; i < n_overlap; i++) {
-        p.band[i].line_offset = off;
+        p.band[i].f_offset = f_offset[i];
+        p.band[i].b_offset = b_offset[i];
         p.band[i].lines = overlap_size[i];
-        off += overlap_size[i];
         if (overlap_size[i] > maxn) maxn = overlap_size[i];
     }
     p.n_bands = n_overlap;

@@ -66,6 +66,7 @@ int fill_params(tops_plan* p)
         r.mom = p->ctx->d_mom;
         r.kaz_sh = p->ctx->d_kaz_sh;
         r.krg64 = p->ctx->d_krg64;
+        r.az_shift_dev = nullptr;
         r.out = it.out;
     }
     p->h_params = prm;

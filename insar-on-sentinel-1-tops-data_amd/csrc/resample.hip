@@ -40,13 +40,20 @@
 
 namespace tops {
 
+// AzimuthShift: a device value (the device-resident ESD flow) or the
+// parameter-block value
+__device__ inline double az_shift_of(const ResampleParams& p)
+{
+    return p.az_shift_dev ? *p.az_shift_dev : p.az_shift;
+}
+
 __device__ inline void slave_pos(const ResampleParams& p, int r, int c, int& iy,
                                  int& fy, int& ix, int& fx)
 {
 #pragma clang fp contract(off)
     double pn = norm_warp((double)(p.mb[0] + c), (double)p.mb[0], (double)p.mb[1]);
     double ln = norm_warp((double)(p.mb[2] + r), (double)p.mb[2], (double)p.mb[3]);
-    double ys = polyval6(ln, pn, p.cpm_az) + p.az_shift;
+    double ys = polyval6(ln, pn, p.cpm_az) + az_shift_of(p);
     double xs = polyval6(ln, pn, p.cpm_rg);
     // keep the int conversion defined for wild polynomials; anything this
     // far out lands in the zero region anyway
@@ -384,6 +391,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
         w.qr = cc; w.qi = s;
     }
 
+    const double azs = az_shift_of(p);
     float2 win[WK_WIN];
     int wbase = INT_MIN;   // absolute slave line held in win[0]
 
@@ -416,7 +424,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
             const double2 ta = t2[0], tb = t2[1], tc = t2[2];
             const double pn = sm.pos[0][lane], q1 = sm.pos[1][lane], q2 = sm.pos[2][lane];
             const double q3 = sm.pos[3][lane], q4 = sm.pos[4][lane];
-            double ys = ta.x + q1 + ta.y + tb.x * pn + q2 + p.az_shift;
+            double ys = ta.x + q1 + ta.y + tb.x * pn + q2 + azs;
             double xs = tb.y + q3 + tc.x + tc.y * pn + q4;
             ys = fmin(fmax(ys, -1.0e9), 1.0e9);
             xs = fmin(fmax(xs, -1.0e9), 1.0e9);

@@ -91,6 +91,7 @@ struct ResampleParams {
     const double* mom;
     const float* kaz_sh;    // tops_ctx::d_kaz_sh
     const double* krg64;    // tops_ctx::d_krg64
+    const double* az_shift_dev;   // device azimuth shift used instead of az_shift, or null
     tops_cf32* out;
 };
 
@@ -128,6 +129,14 @@ int launch_esd(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* sf,
                int pixels, int win_az, int win_rg, float thr, double* d_sums,
                double* d_partials, int max_blocks_per_band, hipStream_t st);
 int esd_partial_capacity(const int* overlap_size, int n_overlap, int pixels);
+// bands at line offsets f_offset[i] (mf, sf) and b_offset[i] (mb, sb)
+int launch_esd_bands(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* sf,
+                     const tops_cf32* sb, const int* f_offset, const int* b_offset,
+                     const int* overlap_size, int n_overlap, int pixels, int win_az, int win_rg,
+                     float thr, double* d_sums, double* d_partials, hipStream_t st);
+// device shifts from the band sums (with the |shift| > 100 reset)
+int launch_esd_shifts(const double* d_sums, int n, double df, double dt, double* d_shifts,
+                      hipStream_t st);
 
 // launchers (geocoreg.hip)
 struct OrbitDev {

@@ -31,6 +31,7 @@ EXPORTS = [
     "tops_deramp_and_resample", "tops_esd_and_coh", "tops_read_config",
     "tops_plan_create", "tops_plan_run", "tops_plan_capture_graph", "tops_plan_tune",
     "tops_plan_strip_rows", "tops_plan_destroy",
+    "tops_subswath_workspace_size", "tops_subswath_esd_coh",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -57,6 +58,16 @@ class OrbitGeom(C.Structure):
 class BurstItem(C.Structure):
     _fields_ = [("geom", BurstGeom), ("slave", _vp), ("kt", _vp), ("tref", _vp), ("fdc", _vp),
                 ("out", _vp), ("master", _vp), ("coh", _vp)]
+
+
+class SubswathDesc(C.Structure):
+    _fields_ = [("n_bursts", C.c_int), ("m_lines_per_burst", C.c_int),
+                ("m_samples_per_burst", C.c_int), ("s_lines_per_burst", C.c_int),
+                ("s_samples_per_burst", C.c_int), ("m_az_time_interval", C.c_double),
+                ("s_az_time_interval", C.c_double), ("spectral_sep", C.c_double),
+                ("overlap_lines", C.POINTER(C.c_int)), ("esd_win_az", C.c_int),
+                ("esd_win_rg", C.c_int), ("threshold", C.c_float), ("coh_win_az", C.c_int),
+                ("coh_win_rg", C.c_int)]
 
 
 class Swath(C.Structure):
@@ -191,6 +202,11 @@ def load():
     L.tops_plan_strip_rows.argtypes = [_vp]
     L.tops_plan_destroy.restype = None
     L.tops_plan_destroy.argtypes = [_vp]
+    L.tops_subswath_workspace_size.restype = C.c_size_t
+    L.tops_subswath_workspace_size.argtypes = [C.POINTER(SubswathDesc)]
+    L.tops_subswath_esd_coh.restype = C.c_int
+    L.tops_subswath_esd_coh.argtypes = [_vp, C.POINTER(SubswathDesc), _vp, _vp, _vp, _vp, _vp,
+                                        _dp, _dp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]
     L.tops_read_config.restype = C.c_int
     L.tops_read_config.argtypes = [C.c_char_p, C.POINTER(Config)]
     _lib = L

@@ -120,3 +120,83 @@ class SubswathRun:
                                        N.ptr(coh, C.c_float), N.ptr(shifts))
         N.check(rc, "tops_esd_and_coh")
         return res, coh, shifts[:n - 1]
+
+
+class DeviceSubswath:
+    """One subswath pair resident in HBM for tops_subswath_esd_coh: the whole
+    ESDAndCoh flow (Func.cpp:2496-2763, pass-1 resample included) as one
+    sequence of launches on one stream, with no host round trip between the
+    ESD shifts and the pass-2 resample.  Device buffers are torch tensors
+    (plumbing only); the work is the library's."""
+
+    def __init__(self, run, ca, cr, esd_win=(6, 24), threshold=0.3, coh_win=(6, 24)):
+        import torch
+        N = native
+        L = N.load()
+        self.torch = torch
+        self.run = run
+        m, s = run.m, run.s
+        nb, lpb, spb = run.nb, run.lpb, run.spb
+        dev = "cuda"
+        put = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+        self.d_master = put(run.m_slc)
+        self.d_slave = put(run.s_slc)
+        so = run.orbits["s"]
+        self.d_kt, self.d_tref, self.d_fdc = (put(np.asarray(so[k], np.float64))
+                                              for k in ("kt", "tref", "fdc"))
+        self.ca = np.ascontiguousarray(ca, dtype=np.float64)
+        self.cr = np.ascontiguousarray(cr, dtype=np.float64)
+        nov = nb - 1
+        self.overlap = np.zeros(max(nov, 1), np.int32)
+        if nov > 0:
+            total, mx = C.c_int(0), C.c_int(0)
+            got = L.tops_overlap_sizes(N.ptr(m.burst_first_line_time),
+                                       N.ptr(m.burst_last_line_time), 0, nb - 1,
+                                       m.az_time_interval, N.ptr(self.overlap, C.c_int),
+                                       C.byref(total), C.byref(mx))
+            assert got == nov, (got, nov)
+        df = L.tops_spectral_sep(N.ptr(run.orbits["m"]["kt"]), m.n_bursts, spb, lpb,
+                                 m.az_time_interval)
+        self.desc = N.SubswathDesc(nb, lpb, spb, s.lines_per_burst, s.samples_per_burst,
+                                   m.az_time_interval, s.az_time_interval, df,
+                                   N.ptr(self.overlap, C.c_int), esd_win[0], esd_win[1],
+                                   threshold, coh_win[0], coh_win[1])
+        ws = L.tops_subswath_workspace_size(C.byref(self.desc))
+        if ws == 0:
+            N.check(-1 if L.tops_last_error() == 0 else L.tops_last_error(),
+                    "tops_subswath_workspace_size")
+        self.workspace = torch.empty(ws, dtype=torch.uint8, device=dev)
+        npx = nb * lpb * spb
+        self.d_res = torch.empty(npx * 8, dtype=torch.uint8, device=dev)
+        self.d_coh = torch.empty(npx * 4, dtype=torch.uint8, device=dev)
+        self.d_shifts = torch.zeros(max(nov, 1), dtype=torch.float64, device=dev)
+        self.ctx = L.tops_ctx_create(N.ptr(run.kaz, C.c_float), N.ptr(run.krg, C.c_float), 12)
+        if not self.ctx:
+            N.check(L.tops_last_error(), "tops_ctx_create")
+
+    def enqueue(self, stream=None):
+        """Launch the flow on `stream` (a torch stream, or the current one)."""
+        N = native
+        st = stream if stream is not None else self.torch.cuda.current_stream()
+        rc = N.load().tops_subswath_esd_coh(
+            self.ctx, C.byref(self.desc), self.d_master.data_ptr(), self.d_slave.data_ptr(),
+            self.d_kt.data_ptr(), self.d_tref.data_ptr(), self.d_fdc.data_ptr(), N.ptr(self.ca),
+            N.ptr(self.cr), self.d_res.data_ptr(), self.d_coh.data_ptr(),
+            self.d_shifts.data_ptr(), self.workspace.data_ptr(), self.workspace.numel(),
+            C.c_void_p(st.cuda_stream))
+        N.check(rc, "tops_subswath_esd_coh")
+
+    def results(self):
+        """(resampled slave [n*L, P] complex64, coherence [n*L, P] float32,
+        ESD shifts [n-1]) copied back to host memory."""
+        self.torch.cuda.synchronize()
+        run = self.run
+        shape = (run.nb * run.lpb, run.spb)
+        res = self.d_res.cpu().numpy().view(np.complex64).reshape(shape)
+        coh = self.d_coh.cpu().numpy().view(np.float32).reshape(shape)
+        return res, coh, self.d_shifts.cpu().numpy()[:run.nb - 1]
+
+    def close(self):
+        if self.ctx:
+            native.load().tops_ctx_destroy(self.ctx)
+            self.ctx = None
