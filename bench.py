@@ -338,6 +338,21 @@ def run_subswath(args, tops, torch):
     torch.cuda.synchronize()
     dev_ms = e0.elapsed_time(e1) / args.steps
     dev_shifts = ds.results()[2]
+    # deburst (Stitch_OneSubswath) of the resident pass-2 slave, CFloat32:
+    # 8 B read + 8 B written per output sample
+    _, height = ds.deburst_extent()
+    stitched = torch.empty(height * run.spb * 8, dtype=torch.uint8, device="cuda")
+    for _ in range(args.warmup):
+        ds.deburst("res", stitched, st)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(args.steps):
+        ds.deburst("res", stitched, st)
+    e1.record(st)
+    torch.cuda.synchronize()
+    db_ms = e0.elapsed_time(e1) / args.steps
+    db_bytes = 16 * height * run.spb
+    del stitched
     ds.close()
     print(json.dumps({
         "metric": METRIC, "value": round(samples / total_s / 1e6, 2), "unit": "Msamples/s",
@@ -350,7 +365,10 @@ def run_subswath(args, tops, torch):
         "stage_ms": ms, "esd_shifts": [round(float(v), 6) for v in out],
         "device_esd_and_coh": {"ms": round(dev_ms, 3),
                                "msamples_per_s": round(samples / dev_ms / 1e3, 1),
-                               "esd_shifts": [round(float(v), 6) for v in dev_shifts]}}))
+                               "esd_shifts": [round(float(v), 6) for v in dev_shifts]},
+        "device_deburst_cf32": {"ms": round(db_ms, 3), "rows": height,
+                                "gb_per_s": round(db_bytes / db_ms / 1e6, 1),
+                                "hbm_frac": round(db_bytes / db_ms / 1e6 / 8000.0, 3)}}))
 
 
 def run_stack(args, tops, torch):

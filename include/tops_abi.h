@@ -275,6 +275,44 @@ int tops_geocoreg(const int32_t* d_dem, int lines, int pixels, double lat0,
                   int xmin, int xmax, int ymin, int ymax, double* cpm_az,
                   double* cpm_rg, int64_t* n_used, double* d_lp, void* stream);
 
+/* Deburst (Stitch_OneSubswath, Func.cpp:2821-2933): the SubSwathInfo
+ * fields it reads (DataTypes.h:51-140).  Times in seconds; the valid-line
+ * times are burstFirstLineTime + firstValidLine * dt and likewise for the
+ * last valid line (Func.cpp:401-403); num_lines is the subswath's
+ * numberOfLines.  At most 16 bursts (the reference's arrays are [16]). */
+typedef struct tops_deburst_swath {
+    int lines_per_burst, samples_per_burst, n_bursts, num_lines;
+    double az_time_interval, first_line_time;
+    const double* burst_first_line_time;        /* host [n_bursts] */
+    const double* burst_last_line_time;         /* host [n_bursts] */
+    const double* burst_first_valid_line_time;  /* host [n_bursts] */
+    const double* burst_last_valid_line_time;   /* host [n_bursts] */
+} tops_deburst_swath;
+
+/* data_type codes of Stitch_OneSubswath (GDAL's GDT_* values). */
+#define TOPS_GDT_FLOAT32 6
+#define TOPS_GDT_CINT16 8
+#define TOPS_GDT_CFLOAT32 10
+
+/* Output extent of the deburst of bursts burst0 .. burstN: first subswath
+ * line y_min and row count (yMin, yMax of Func.cpp:2837-2838).  Fails
+ * (TOPS_ERR_ARG) where an output line time lies in no burst: the reference
+ * exits there (Func.cpp:2868-2876). */
+int tops_deburst_extent(const tops_deburst_swath* sw, int burst0, int burstN, int* y_min,
+                        int* height);
+
+/* Row of the burst-stacked input each output row copies
+ * (getLineIdxInMetaData, Func.cpp:2766-2819, minus burst0 * lines per
+ * burst); rows outside 0 .. (burstN-burst0+1)*lpb-1 give zero lines.
+ * rows: host [height]. */
+int tops_deburst_rows(const tops_deburst_swath* sw, int burst0, int burstN, int* rows);
+
+/* Stitch_OneSubswath on rasters in HBM: d_gapped [(burstN-burst0+1)*lpb x
+ * spb] of data_type elements -> d_out [height x spb].  One launch on
+ * `stream`; nothing is allocated (capturable). */
+int tops_deburst(const tops_deburst_swath* sw, int burst0, int burstN, int data_type,
+                 const void* d_gapped, void* d_out, void* stream);
+
 /* ======================================================================== */
 /* Part 3: host stage layer                                                 */
 /* ======================================================================== */
@@ -381,6 +419,12 @@ typedef struct {
     int firstsubswath, lastsubswath, multi_az, multi_rg, burst0, burstN;
 } tops_config;
 int tops_read_config(const char* path, tops_config* cfg);
+
+/* Replaces Stitch_OneSubswath, Func.cpp:2821-2933, with host buffers for
+ * its two TIFF files: gapped [(burstN-burst0+1)*lpb x spb] -> out
+ * [height x spb] (tops_deburst_extent), through the device kernel. */
+int tops_stitch_subswath(const tops_deburst_swath* sw, int burst0, int burstN, int data_type,
+                         const void* gapped, void* out);
 
 #ifdef __cplusplus
 }

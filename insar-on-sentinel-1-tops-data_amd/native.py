@@ -18,6 +18,9 @@ TOPS_ERR_NODEVICE = 4
 TOPS_ERR_ALLOC = 5
 MODE_FAST = 0
 MODE_FP64 = 1
+GDT_FLOAT32 = 6
+GDT_CINT16 = 8
+GDT_CFLOAT32 = 10
 
 # every symbol declared in include/tops_abi.h
 EXPORTS = [
@@ -32,6 +35,7 @@ EXPORTS = [
     "tops_plan_create", "tops_plan_run", "tops_plan_capture_graph", "tops_plan_tune",
     "tops_plan_strip_rows", "tops_plan_destroy",
     "tops_subswath_workspace_size", "tops_subswath_esd_coh",
+    "tops_deburst_extent", "tops_deburst_rows", "tops_deburst", "tops_stitch_subswath",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -68,6 +72,14 @@ class SubswathDesc(C.Structure):
                 ("overlap_lines", C.POINTER(C.c_int)), ("esd_win_az", C.c_int),
                 ("esd_win_rg", C.c_int), ("threshold", C.c_float), ("coh_win_az", C.c_int),
                 ("coh_win_rg", C.c_int)]
+
+
+class DeburstSwath(C.Structure):
+    _fields_ = [("lines_per_burst", C.c_int), ("samples_per_burst", C.c_int),
+                ("n_bursts", C.c_int), ("num_lines", C.c_int), ("az_time_interval", C.c_double),
+                ("first_line_time", C.c_double), ("burst_first_line_time", _dp),
+                ("burst_last_line_time", _dp), ("burst_first_valid_line_time", _dp),
+                ("burst_last_valid_line_time", _dp)]
 
 
 class Swath(C.Structure):
@@ -207,6 +219,15 @@ def load():
     L.tops_subswath_esd_coh.restype = C.c_int
     L.tops_subswath_esd_coh.argtypes = [_vp, C.POINTER(SubswathDesc), _vp, _vp, _vp, _vp, _vp,
                                         _dp, _dp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]
+    _dsw = C.POINTER(DeburstSwath)
+    L.tops_deburst_extent.restype = C.c_int
+    L.tops_deburst_extent.argtypes = [_dsw, C.c_int, C.c_int, _ip, _ip]
+    L.tops_deburst_rows.restype = C.c_int
+    L.tops_deburst_rows.argtypes = [_dsw, C.c_int, C.c_int, _ip]
+    L.tops_deburst.restype = C.c_int
+    L.tops_deburst.argtypes = [_dsw, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]
+    L.tops_stitch_subswath.restype = C.c_int
+    L.tops_stitch_subswath.argtypes = [_dsw, C.c_int, C.c_int, C.c_int, _vp, _vp]
     L.tops_read_config.restype = C.c_int
     L.tops_read_config.argtypes = [C.c_char_p, C.POINTER(Config)]
     _lib = L

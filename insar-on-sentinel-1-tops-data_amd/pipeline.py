@@ -70,6 +70,17 @@ class SubswathRun:
         return N.Orbit(len(o["t"]), N.ptr(o["t"]), len(o["cx"]), N.ptr(o["cx"]), N.ptr(o["cy"]),
                        N.ptr(o["cz"]))
 
+    def deburst_swath(self, which="m"):
+        """tops_deburst_swath of the master (or slave) subswath."""
+        sw = self.m if which == "m" else self.s
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                (sw.burst_first_line_time, sw.burst_last_line_time,
+                 sw.burst_first_valid_line_time, sw.burst_last_valid_line_time)]
+        self._keep.append(arrs)
+        return native.DeburstSwath(sw.lines_per_burst, sw.samples_per_burst, sw.n_bursts,
+                                   sw.num_lines, sw.az_time_interval, sw.first_line_time,
+                                   *[native.ptr(a) for a in arrs])
+
     def dem_struct(self):
         lat0, lon0, dlat, dlon = self.dem_geo
         return native.Dem(self.dem.ctypes.data, self.dem.shape[0], self.dem.shape[1], lat0, lon0,
@@ -120,6 +131,31 @@ class SubswathRun:
                                        N.ptr(coh, C.c_float), N.ptr(shifts))
         N.check(rc, "tops_esd_and_coh")
         return res, coh, shifts[:n - 1]
+
+    def stitch(self, raster, b0=0, bn=None):
+        """Stitch_OneSubswath (Func.cpp:2821-2933) of a burst-stacked host
+        raster of bursts b0..bn (CInt16 [.., P, 2] int16, complex64 or
+        float32) onto the master's continuous line grid."""
+        N = native
+        L = N.load()
+        bn = self.nb - 1 if bn is None else bn
+        raster = np.ascontiguousarray(raster)
+        if raster.dtype == np.int16:
+            dtype = N.GDT_CINT16
+        elif raster.dtype == np.complex64:
+            dtype = N.GDT_CFLOAT32
+        elif raster.dtype == np.float32:
+            dtype = N.GDT_FLOAT32
+        else:
+            raise TypeError(f"unsupported raster type {raster.dtype}")
+        sw = self.deburst_swath()
+        y0, h = C.c_int(0), C.c_int(0)
+        N.check(L.tops_deburst_extent(C.byref(sw), b0, bn, C.byref(y0), C.byref(h)),
+                "tops_deburst_extent")
+        out = np.empty((h.value,) + raster.shape[1:], raster.dtype)
+        N.check(L.tops_stitch_subswath(C.byref(sw), b0, bn, dtype, raster.ctypes.data,
+                                       out.ctypes.data), "tops_stitch_subswath")
+        return out
 
 
 class DeviceSubswath:
@@ -195,6 +231,27 @@ class DeviceSubswath:
         res = self.d_res.cpu().numpy().view(np.complex64).reshape(shape)
         coh = self.d_coh.cpu().numpy().view(np.float32).reshape(shape)
         return res, coh, self.d_shifts.cpu().numpy()[:run.nb - 1]
+
+    def deburst_extent(self, b0=0, bn=None):
+        bn = self.run.nb - 1 if bn is None else bn
+        y0, h = C.c_int(0), C.c_int(0)
+        sw = self.run.deburst_swath()
+        native.check(native.load().tops_deburst_extent(C.byref(sw), b0, bn, C.byref(y0),
+                                                       C.byref(h)), "tops_deburst_extent")
+        return y0.value, h.value
+
+    def deburst(self, which, out, stream=None):
+        """Stitch the resident resampled slave ("res"), coherence ("coh") or
+        master ("master") of every burst into `out` (a device tensor of
+        height x P elements) on `stream`."""
+        N = native
+        src, dtype = {"res": (self.d_res, N.GDT_CFLOAT32), "coh": (self.d_coh, N.GDT_FLOAT32),
+                      "master": (self.d_master, N.GDT_CINT16)}[which]
+        st = stream if stream is not None else self.torch.cuda.current_stream()
+        sw = self.run.deburst_swath()
+        N.check(N.load().tops_deburst(C.byref(sw), 0, self.run.nb - 1, dtype, src.data_ptr(),
+                                      out.data_ptr(), C.c_void_p(st.cuda_stream)),
+                "tops_deburst")
 
     def close(self):
         if self.ctx:

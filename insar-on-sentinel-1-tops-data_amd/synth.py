@@ -163,6 +163,19 @@ class Swath:
     scene_line0: np.ndarray          # absolute scene line of each burst's first line
     first_line_time: float = 0.0
     last_line_time: float = 0.0
+    # first / last valid line of each burst (the annotation's firstValidSample
+    # rows that are not -1, Func.cpp:370-406) and the subswath's line count
+    first_valid_line: np.ndarray = None
+    last_valid_line: np.ndarray = None
+    num_lines: int = 0
+
+    @property
+    def burst_first_valid_line_time(self):
+        return self.burst_first_line_time + self.first_valid_line * self.az_time_interval
+
+    @property
+    def burst_last_valid_line_time(self):
+        return self.burst_first_line_time + self.last_valid_line * self.az_time_interval
 
 
 def deramp_params(sw, speed):
@@ -303,6 +316,8 @@ def _swath(name_seed, lines, pixels, n_bursts, t_first, cycle_lines, slr0, dc_sl
     # a hair less than the cycle so EstOverlapSize's floor is not marginal
     first = first - np.arange(n_bursts) * 1e-9
     last = first + (lines - 1) * dt
+    # invalid lines at each end of a burst, as in IW annotations
+    margin = min(lines // 30, 40)
     two_way0 = 2 * slr0
     drange = 2 * RANGE_SPACING / SOL
     c1 = dc_slope_total / (pixels - 1) / drange
@@ -324,7 +339,10 @@ def _swath(name_seed, lines, pixels, n_bursts, t_first, cycle_lines, slr0, dc_sl
                  first_valid_pixel=0, burst_first_line_time=first, burst_last_line_time=last,
                  dc=dc, fm=fm, burst_ll=ll,
                  scene_line0=np.arange(n_bursts, dtype=np.int64) * cycle_lines,
-                 first_line_time=float(first[0]), last_line_time=float(last[-1]))
+                 first_line_time=float(first[0]), last_line_time=float(last[-1]),
+                 first_valid_line=np.full(n_bursts, margin, dtype=np.int64),
+                 last_valid_line=np.full(n_bursts, lines - 1 - margin, dtype=np.int64),
+                 num_lines=n_bursts * lines)
 
 
 def make_pair(name, lines, pixels, n_bursts=1, overlap=100, seed=20260101, stream=0,

@@ -71,6 +71,15 @@ def load():
                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.orc_spectral_sep.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double]
     L.orc_spectral_sep.restype = C.c_double
+    L.orc_deburst_line.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                   C.c_int, C.c_int]
+    L.orc_deburst_line.restype = C.c_int
+    L.orc_deburst_extent.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.orc_deburst.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                              C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                              C.POINTER(C.c_int)]
+    L.orc_deburst.restype = C.c_int
     _lib = L
     return L
 
@@ -198,3 +207,46 @@ def overlap_sizes(first, last, b0, bn, dt):
 def spectral_sep(kt, lpb, dt):
     kt = np.ascontiguousarray(kt, dtype=np.float64)
     return load().orc_spectral_sep(_p(kt), kt.shape[0], kt.shape[1], lpb, dt)
+
+
+def _times(a):
+    return _p(np.ascontiguousarray(a, dtype=np.float64))
+
+
+def deburst_line(first, last, lpb, num_lines, dt, t, b0, bn):
+    """getLineIdxInMetaData (Func.cpp:2766-2819)"""
+    return load().orc_deburst_line(_times(first), _times(last), len(first), lpb, num_lines, dt, t,
+                                   b0, bn)
+
+
+def deburst_extent(first_valid_b0, last_valid_bn, first_line_time, dt):
+    y0, h = C.c_int(0), C.c_int(0)
+    load().orc_deburst_extent(first_valid_b0, last_valid_bn, first_line_time, dt, C.byref(y0),
+                              C.byref(h))
+    return y0.value, h.value
+
+
+def deburst(first, last, first_valid, last_valid, lpb, num_lines, dt, first_line_time, b0, bn,
+            gapped=None):
+    """Stitch_OneSubswath (Func.cpp:2821-2933) over the burst-stacked raster
+    `gapped` [(bn-b0+1)*lpb, P] of 4- or 8-byte elements (or None: rows
+    only).  Returns (stitched raster or None, gapped row of each output row),
+    or (None, None) where a line time falls in no burst."""
+    first, last = (np.ascontiguousarray(a, dtype=np.float64) for a in (first, last))
+    fv, lv = (np.ascontiguousarray(a, dtype=np.float64) for a in (first_valid, last_valid))
+    _, h = deburst_extent(fv[b0], lv[bn], first_line_time, dt)
+    rows = np.zeros(max(h, 1), np.int32)
+    if gapped is not None:
+        gapped = np.ascontiguousarray(gapped)
+        assert gapped.shape[0] == (bn - b0 + 1) * lpb
+        out = np.zeros((max(h, 0),) + gapped.shape[1:], gapped.dtype)
+        pixels, eb = gapped.shape[1], gapped.strides[1]
+        gp, op = gapped.ctypes.data, out.ctypes.data
+    else:
+        out, pixels, eb, gp, op = None, 0, 4, None, None
+    rc = load().orc_deburst(_p(first), _p(last), _p(fv), _p(lv), len(first), lpb, pixels,
+                            num_lines, dt, first_line_time, b0, bn, gp, eb, op,
+                            rows.ctypes.data_as(C.POINTER(C.c_int)))
+    if rc != 0:
+        return None, None
+    return out, rows[:h]

@@ -587,3 +587,86 @@ double orc_spectral_sep(const double* kt, int n_bursts, int spb,
         for (int p = 0; p < spb; p++) sum += kt[i * spb + p] * tcycle;
     return sum / spb / n_bursts;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Deburst: getLineIdxInMetaData (Func.cpp:2766-2819) and                    */
+/* Stitch_OneSubswath (Func.cpp:2821-2933)                                   */
+/* ------------------------------------------------------------------------ */
+
+/* Func.cpp:2766-2819: subswath line holding line time t; -1 when none. */
+int orc_deburst_line(const double* first_line_time, const double* last_line_time,
+                     int n_bursts, int lines_per_burst, int num_lines,
+                     double az_time_interval, double t, int burst0, int burstN)
+{
+    int sy0 = -1, sy1 = -1, b0 = -1, b1 = -1, k = 0;
+    for (int i = 0; i < n_bursts; i++) {
+        if (!(t >= first_line_time[i] && t < last_line_time[i])) continue;
+        int sy = i * lines_per_burst +
+                 (int)(((t - first_line_time[i]) / az_time_interval) + 0.5);
+        if (k == 0) {
+            sy0 = sy;
+            b0 = i;
+        } else {
+            sy1 = sy;
+            b1 = i;
+            break;
+        }
+        k++;
+    }
+    if (sy0 != -1 && sy1 != -1) {
+        if (b1 == burst0) return sy1;
+        if (b1 == burstN + 1) return sy0;
+        double mid = (last_line_time[b0] + first_line_time[b1]) / 2.0;
+        if (t > mid) return sy1;
+    }
+    double tl = last_line_time[n_bursts - 1];
+    if (t < tl + az_time_interval && t >= tl) sy0 = num_lines - 1;
+    return sy0;
+}
+
+/* Func.cpp:2825-2843: output rows yMin .. yMax. */
+void orc_deburst_extent(double first_valid_time_b0, double last_valid_time_bN,
+                        double first_line_time, double az_time_interval,
+                        int* y_min, int* height)
+{
+    int ymin = (int)((first_valid_time_b0 - first_line_time) / az_time_interval);
+    int ymax = (int)((last_valid_time_bN - first_line_time) / az_time_interval);
+    *y_min = ymin;
+    *height = ymax - ymin + 1;
+}
+
+/* Func.cpp:2864-2915: out row yy = gapped row (line - burst0 * lpb), or zeros
+ * where that row is outside the gapped raster (the failed read leaves the
+ * memset line).  Returns -1 (the reference exits) if a line time falls in
+ * no burst, else 0.  rows (may be NULL) receives the gapped row of each
+ * output row. */
+int orc_deburst(const double* first_line_time, const double* last_line_time,
+                const double* first_valid_time, const double* last_valid_time,
+                int n_bursts, int lines_per_burst, int samples_per_burst,
+                int num_lines, double az_time_interval, double swath_first_line_time,
+                int burst0, int burstN, const void* gapped, int elem_bytes,
+                void* out, int* rows)
+{
+    int ymin, height;
+    orc_deburst_extent(first_valid_time[burst0], last_valid_time[burstN],
+                       swath_first_line_time, az_time_interval, &ymin, &height);
+    const double tf = first_valid_time[burst0];
+    const int gapped_rows = (burstN - burst0 + 1) * lines_per_burst;
+    const size_t row_bytes = (size_t)samples_per_burst * elem_bytes;
+    for (int yy = 0; yy < height; yy++) {
+        double t = tf + yy * az_time_interval;
+        int line = orc_deburst_line(first_line_time, last_line_time, n_bursts,
+                                    lines_per_burst, num_lines, az_time_interval, t,
+                                    burst0, burstN);
+        if (line == -1) return -1;
+        line -= burst0 * lines_per_burst;
+        if (rows) rows[yy] = line;
+        if (!out) continue;
+        char* dst = (char*)out + (size_t)yy * row_bytes;
+        if (line >= 0 && line < gapped_rows)
+            memcpy(dst, (const char*)gapped + (size_t)line * row_bytes, row_bytes);
+        else
+            memset(dst, 0, row_bytes);
+    }
+    return 0;
+}

@@ -127,6 +127,21 @@ void orc_overlap_sizes(const double* first_line_time,
 double orc_spectral_sep(const double* kt, int n_bursts, int spb,
                         int lines_per_burst, double az_time_interval);
 
+/* --- deburst: getLineIdxInMetaData (Func.cpp:2766-2819), the extent and
+ * row loop of Stitch_OneSubswath (Func.cpp:2821-2933).                     */
+int orc_deburst_line(const double* first_line_time, const double* last_line_time,
+                     int n_bursts, int lines_per_burst, int num_lines,
+                     double az_time_interval, double t, int burst0, int burstN);
+void orc_deburst_extent(double first_valid_time_b0, double last_valid_time_bN,
+                        double first_line_time, double az_time_interval,
+                        int* y_min, int* height);
+int orc_deburst(const double* first_line_time, const double* last_line_time,
+                const double* first_valid_time, const double* last_valid_time,
+                int n_bursts, int lines_per_burst, int samples_per_burst,
+                int num_lines, double az_time_interval, double swath_first_line_time,
+                int burst0, int burstN, const void* gapped, int elem_bytes,
+                void* out, int* rows);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,3 +109,22 @@ def test_subswath_device_graph_replay(dev, flow):
         assert np.array_equal(res, flow["res"]), rep
         assert np.array_equal(coh, flow["coh"]), rep
     del g
+
+
+def test_subswath_device_deburst(orc, dev, run, flow):
+    """The resident coherence, resampled slave and master stitched on the
+    device (tops_deburst) equal the oracle's stitch of the same rasters."""
+    ds = flow["ds"]
+    y0, h = ds.deburst_extent()
+    m = run.m
+    arrs = (m.burst_first_line_time, m.burst_last_line_time, m.burst_first_valid_line_time,
+            m.burst_last_valid_line_time)
+    for which, src, eb in (("coh", flow["coh"], 4), ("res", flow["res"], 8),
+                           ("master", run.m_slc, 4)):
+        out = dev.empty(h * run.spb * eb)
+        ds.deburst(which, out)
+        got = dev.get(out, src.dtype, (h,) + src.shape[1:])
+        ref, _ = orc.deburst(*arrs, run.lpb, m.num_lines, m.az_time_interval, m.first_line_time,
+                             0, run.nb - 1, gapped=src)
+        print(f"  deburst {which}: {h} rows from line {y0}", flush=True)
+        assert np.array_equal(got, ref)

@@ -116,3 +116,20 @@ def test_full_pass2_resample_and_coherence(orc, full):
         print(f"  burst {b}: pass-2 {er:.2e}, coherence {ec:.2e}", flush=True)
     assert np.array_equal(full["res2"][:run.lpb], full["res1"][:run.lpb])
     assert worst_r <= 1e-5 and worst_c <= 1e-4
+
+
+def test_full_deburst(orc, full):
+    """Stitch_OneSubswath of the pass-2 slave (CFloat32), the coherence
+    (Float32) and the master (CInt16) of all 9 bursts, bit-exact against the
+    oracle's stitch."""
+    run = full["run"]
+    m = run.m
+    arrs = (m.burst_first_line_time, m.burst_last_line_time, m.burst_first_valid_line_time,
+            m.burst_last_valid_line_time)
+    for name, src in (("res2", full["res2"]), ("coh", full["coh"]), ("master", run.m_slc)):
+        got = run.stitch(src)
+        ref, _ = orc.deburst(*arrs, run.lpb, m.num_lines, m.az_time_interval, m.first_line_time,
+                             0, run.nb - 1, gapped=src)
+        print(f"  stitch {name}: {got.shape}", flush=True)
+        assert np.array_equal(got, ref)
+        del got, ref
