@@ -7,7 +7,9 @@
  * unchanged.  std::complex<short> is laid out as tops_ci16 and cuComplex /
  * std::complex<float> as tops_cf32.  Differences in behaviour:
  *   - no exit() and no device reset inside the library; errors set a status
- *     readable with tops_last_error() and are printed once to stderr;
+ *     readable with tops_last_error() and are printed once to stderr (each
+ *     Part 1 entry point clears the status on entry, so a check after the
+ *     call reports that call);
  *   - per-call parameter blocks instead of module globals (reentrant);
  *   - device buffers returned by DerampDemodResample_ESD/ResampleFirstBurst
  *     are dense [rows x cols] (pitch = cols) and are released by the

@@ -473,6 +473,7 @@ void DerampDemodResample(tops_ci16* SlaveArray, double* CpmAz, double* CpmRg, do
                          double* dopplerRate, double* referenceTime, double* dopplerCentroid,
                          int Npoints)
 {
+    tops_clear_error();
     (void)slave_pixels;
     (void)slave_lines;
     ref_resample(SlaveArray, CpmAz, CpmRg, AzimuthShift, output, KernelAz, KernelRg, sBurstIdx,
@@ -488,6 +489,7 @@ tops_cf32* DerampDemodResample_ESD(tops_ci16* SlaveArray, double* CpmAz, double*
                                    double azimuthTimeInterval, double* dopplerRate,
                                    double* referenceTime, double* dopplerCentroid, int Npoints)
 {
+    tops_clear_error();
     (void)slave_pixels;
     (void)slave_lines;
     return ref_resample(SlaveArray, CpmAz, CpmRg, AzimuthShift, output, KernelAz, KernelRg,
@@ -498,6 +500,7 @@ tops_cf32* DerampDemodResample_ESD(tops_ci16* SlaveArray, double* CpmAz, double*
 
 tops_cf32* ResampleFirstBurst(tops_cf32* SlaveArray, int ww, int hh)
 {
+    tops_clear_error();
     if (!SlaveArray || ww <= 0 || hh <= 0) {
         set_error(TOPS_ERR_ARG, "ResampleFirstBurst: bad arguments");
         return nullptr;
@@ -517,6 +520,7 @@ tops_cf32* ResampleFirstBurst(tops_cf32* SlaveArray, int ww, int hh)
 void CoherenceEst(int Dx0, int Dy0, int Dw, int Dh, int dx, int dy, int cohWinRg, int cohWinAz,
                   tops_ci16* masterArray, tops_cf32* d_slaveArray, float* cohdata)
 {
+    tops_clear_error();
     (void)Dx0;
     (void)Dy0;
     ref_coherence(Dw, Dh, dx, dy, cohWinRg, cohWinAz, masterArray, d_slaveArray, cohdata);
@@ -526,6 +530,7 @@ void CoherenceEst_ESD(int Dx0, int Dy0, int Dw, int Dh, int dx, int dy, int cohW
                       int cohWinAz, tops_ci16* masterArray, tops_cf32* d_slaveArray,
                       float* cohdata)
 {
+    tops_clear_error();
     (void)Dx0;
     (void)Dy0;
     ref_coherence(Dw, Dh, dx, dy, cohWinRg, cohWinAz, masterArray, d_slaveArray, cohdata);
@@ -536,6 +541,7 @@ void EstESDShifts(tops_ci16* MasterFor, tops_ci16* MasterBack, tops_cf32* SlaveF
                   int ArrayLines, int ArrayPixels, int CohWinAz, int CohWinRg, int OverlapMaxLines,
                   float CohThresHold, double spectralSeparation, double azimuthTimeInterval)
 {
+    tops_clear_error();
     (void)OverlapMaxLines;
     if (numOverlap <= 0) return;
     if (!MasterFor || !MasterBack || !SlaveFor || !SlaveBack || !OverlapSizeArray || !ShiftArray) {
@@ -581,6 +587,7 @@ void GeoCoreg_warp_cuda(int m_burstIdx, int s_burstIdx, int xmin, int xmax, int 
                         double* m_y_coef, double* m_z_coef, double* s_x_coef, double* s_y_coef,
                         double* s_z_coef, double m_aziInitial, double s_aziInitial)
 {
+    tops_clear_error();
     (void)wavelength;
     (void)m_numOfSamples;
     (void)s_numOfSamples;

@@ -16,11 +16,14 @@
 // WR products (no running subtraction, so the rounding error stays relative
 // to each window's own sum of magnitudes).  The WA latest rows of range sums
 // sit in a rolling register buffer; an output row is their direct sum.  The
-// next row's samples are requested before the current row is summed.
+// next row's samples are requested before the current row is summed.  The
+// pair loads and the output stores are nontemporal (each sample is read by
+// one workgroup, each output written once).
 #pragma once
 
-#include "tops_internal.hpp"
+#include "stream_access.hpp"
 #include "tile_window.hpp"
+#include "tops_internal.hpp"
 
 namespace tops {
 
@@ -75,10 +78,10 @@ struct CohWalk {
         const int* mi = reinterpret_cast<const int*>(M);
         if (vec && c + 1 < w) {
             // rb + c is even: the pair loads are aligned
-            const int2 m2 = *reinterpret_cast<const int2*>(mi + rb + c);
+            const int2 m2 = stream_load_int2(mi + rb + c);
             raw.m0 = m2.x;
             raw.m1 = m2.y;
-            raw.s = *reinterpret_cast<const float4*>(S + rb + c);
+            raw.s = stream_load_float4(S + rb + c);
         } else {
             if (c < w) {
                 raw.m0 = mi[rb + c];
@@ -158,7 +161,7 @@ struct CohWalk {
                 }
                 float* dst = coh + (size_t)r * w + c;
                 if (vec && c + 1 < w) {
-                    *reinterpret_cast<float2*>(dst) = make_float2(out[0], out[1]);
+                    stream_store_float2(dst, out[0], out[1]);
                 } else {
                     if (c < w) dst[0] = out[0];
                     if (c + 1 < w) dst[1] = out[1];

@@ -5,111 +5,97 @@
 // copied from the burst that owns its time (getLineIdxInMetaData,
 // Func.cpp:2766-2819).
 //
-// The work is a row gather: HBM-bound, 2 x row bytes per output row (1 x for
-// the zero rows the reference leaves where a source row lies outside the
-// input).  One launch covers the whole output; each workgroup moves one
-// 16 KiB chunk of one output row and evaluates that row's source line
-// itself from the burst times it receives by value, so the launch needs no
-// table in device memory and can be captured in a hipGraph.
-#include "deburst.hpp"
+// The host evaluates the line map of every output row (it must anyway: the
+// reference exits on a line time that no burst holds) and hands the kernel
+// the map as runs of rows with a constant source offset -- one run per burst
+// in practice -- by value.  The device work is then a row gather with no
+// arithmetic: HBM-bound, 2 x row bytes per output row (1 x for the zero rows
+// the reference leaves where a source row lies outside the input), one
+// 16-byte vector per thread, nontemporal loads and stores (each row is
+// touched once).  Measured 0.66 ms for 12619 rows of 21000 CFloat32
+// (6.4 TB/s), the rate of a plain streaming copy; evaluating the FP64 map
+// per workgroup instead took 0.94 ms.
+#include "tops_internal.hpp"
 
 #include <climits>
 #include <stdint.h>
 #include <string.h>
 
+#include <vector>
+
 namespace tops {
 namespace {
 
-constexpr int kDbThreads = 256;
-constexpr int kDbUnroll = 4;        // vectors in flight per thread
+// Stitch_OneSubswath's SubSwathInfo arrays are [16] (DataTypes.h:95-96).
+constexpr int kMaxBursts = 16;
 
-typedef unsigned int DbVec16 __attribute__((ext_vector_type(4)));
+struct LineMap {
+    double first[kMaxBursts];   // burstFirstLineTime
+    double last[kMaxBursts];    // burstLastLineTime
+    double t_first;             // burstFirstValidLineTime[burst0]
+    double dt;                  // azimuthTimeInterval
+    int n_bursts, lines_per_burst, num_lines, burst0, burst_n;
+    int height;                 // output rows
+    int gapped_rows;            // rows of the burst-stacked input
+};
 
-template <typename V>
-__global__ __launch_bounds__(kDbThreads) void deburst_kernel(DeburstMap m, const V* src, V* dst,
-                                                             long row_vecs, int chunks)
+// getLineIdxInMetaData (Func.cpp:2766-2819): the subswath line whose time
+// is nearest t, taken from the burst that owns t; in an overlap, the earlier
+// burst up to the mid time between its last line and the next burst's first
+// line (the burst inside burst0..burstN when the other is outside).  -1 when
+// no burst holds t (the last line of the last burst and one interval after
+// it map to the subswath's last line).
+int line_of_time(const LineMap& m, double t)
 {
-    const int yy = (int)(blockIdx.x / (unsigned)chunks);
-    const int chunk = (int)(blockIdx.x - (unsigned)yy * (unsigned)chunks);
-    const int row = deburst_source_row(m, yy);          // the same for the whole workgroup
-    V* out = dst + (size_t)yy * row_vecs;
-    const long v0 = (long)chunk * (kDbThreads * kDbUnroll) + threadIdx.x;
-    if (row >= 0 && row < m.gapped_rows) {
-        const V* in = src + (size_t)row * row_vecs;
-        V v[kDbUnroll];
-#pragma unroll
-        for (int k = 0; k < kDbUnroll; k++) {
-            const long i = v0 + (long)k * kDbThreads;
-            if (i < row_vecs) v[k] = in[i];
-        }
-#pragma unroll
-        for (int k = 0; k < kDbUnroll; k++) {
-            const long i = v0 + (long)k * kDbThreads;
-            if (i < row_vecs) out[i] = v[k];
-        }
-    } else {
-        // the reference's read of a row outside the gapped raster fails and
-        // leaves the zeroed line buffer (Func.cpp:2880-2883)
-        const V z = {};
-#pragma unroll
-        for (int k = 0; k < kDbUnroll; k++) {
-            const long i = v0 + (long)k * kDbThreads;
-            if (i < row_vecs) out[i] = z;
+#pragma clang fp contract(off)
+    int sy0 = -1, sy1 = -1, b0 = -1, b1 = -1;
+    for (int i = 0; i < m.n_bursts; i++) {
+        if (!(t >= m.first[i] && t < m.last[i])) continue;
+        const int sy = i * m.lines_per_burst + (int)(((t - m.first[i]) / m.dt) + 0.5);
+        if (b0 == -1) {
+            sy0 = sy;
+            b0 = i;
+        } else {
+            sy1 = sy;
+            b1 = i;
+            break;
         }
     }
-}
-
-}  // namespace
-
-int launch_deburst(const DeburstMap& m, const void* src, void* dst, size_t row_bytes,
-                   hipStream_t s)
-{
-    if (m.height <= 0 || row_bytes == 0) return TOPS_OK;
-    const bool v16 = row_bytes % 16 == 0 && (uintptr_t)src % 16 == 0 && (uintptr_t)dst % 16 == 0;
-    if (!v16 && (row_bytes % 4 != 0 || (uintptr_t)src % 4 != 0 || (uintptr_t)dst % 4 != 0))
-        return set_error(TOPS_ERR_ARG, "deburst: rasters must be 4-byte aligned");
-    const size_t vec = v16 ? 16 : 4;
-    const size_t row_vecs = row_bytes / vec;
-    const size_t per = (size_t)kDbThreads * kDbUnroll;
-    const size_t chunks = (row_vecs + per - 1) / per;
-    const size_t blocks = chunks * (size_t)m.height;
-    if (blocks > (size_t)INT_MAX) return set_error(TOPS_ERR_ARG, "deburst: output too large");
-    if (v16)
-        deburst_kernel<DbVec16><<<dim3((unsigned)blocks), dim3(kDbThreads), 0, s>>>(
-            m, (const DbVec16*)src, (DbVec16*)dst, (long)row_vecs, (int)chunks);
-    else
-        deburst_kernel<unsigned int><<<dim3((unsigned)blocks), dim3(kDbThreads), 0, s>>>(
-            m, (const unsigned int*)src, (unsigned int*)dst, (long)row_vecs, (int)chunks);
-    TOPS_HIP(hipGetLastError());
-    return TOPS_OK;
-}
-
-namespace {
-
-int element_bytes(int data_type)
-{
-    switch (data_type) {
-    case TOPS_GDT_CINT16: return 4;
-    case TOPS_GDT_FLOAT32: return 4;
-    case TOPS_GDT_CFLOAT32: return 8;
-    default: return 0;
+    if (sy0 != -1 && sy1 != -1) {
+        if (b1 == m.burst0) return sy1;
+        if (b1 == m.burst_n + 1) return sy0;
+        if (t > (m.last[b0] + m.first[b1]) / 2.0) return sy1;
     }
+    const double tl = m.last[m.n_bursts - 1];
+    if (t < tl + m.dt && t >= tl) sy0 = m.num_lines - 1;
+    return sy0;
 }
 
-// the extent of Func.cpp:2825-2843 and the burst times; every output row
-// checked to have a source line (the reference exits where one has none,
-// Func.cpp:2868-2876)
-int make_map(const tops_deburst_swath* sw, int burst0, int burstN, DeburstMap* m, int* y_min,
-             const char* who)
+// row of the burst-stacked input that output row yy copies (outside
+// 0 .. gapped_rows-1: a zero row); INT_MIN when its line time is in no burst
+int source_row(const LineMap& m, int yy)
+{
+#pragma clang fp contract(off)
+    const double t = m.t_first + yy * m.dt;
+    const int line = line_of_time(m, t);
+    if (line == -1) return INT_MIN;
+    return line - m.burst0 * m.lines_per_burst;
+}
+
+// the extent of Func.cpp:2825-2843, the burst times and the source row of
+// every output row; fails where a line time lies in no burst (the reference
+// exits there, Func.cpp:2868-2876)
+int make_map(const tops_deburst_swath* sw, int burst0, int burstN, LineMap* m, int* y_min,
+             std::vector<int>* rows, const char* who)
 {
 #pragma clang fp contract(off)
     if (!sw || !sw->burst_first_line_time || !sw->burst_last_line_time ||
         !sw->burst_first_valid_line_time || !sw->burst_last_valid_line_time ||
         sw->lines_per_burst <= 0 || sw->samples_per_burst <= 0 || !(sw->az_time_interval > 0))
         return set_error(TOPS_ERR_ARG, "%s: bad subswath description", who);
-    if (sw->n_bursts < 1 || sw->n_bursts > kDeburstMaxBursts)
+    if (sw->n_bursts < 1 || sw->n_bursts > kMaxBursts)
         return set_error(TOPS_ERR_ARG, "%s: %d bursts (1 .. %d supported)", who, sw->n_bursts,
-                         kDeburstMaxBursts);
+                         kMaxBursts);
     if (burst0 < 0 || burstN < burst0 || burstN >= sw->n_bursts)
         return set_error(TOPS_ERR_ARG, "%s: burst range %d .. %d of %d", who, burst0, burstN,
                          sw->n_bursts);
@@ -132,12 +118,106 @@ int make_map(const tops_deburst_swath* sw, int burst0, int burstN, DeburstMap* m
     if (y_min) *y_min = ymin;
     if (m->height <= 0)
         return set_error(TOPS_ERR_ARG, "%s: empty output (%d rows)", who, m->height);
-    for (int yy = 0; yy < m->height; yy++)
-        if (deburst_source_row(*m, yy) == INT_MIN)
+    if (rows) rows->resize(m->height);
+    for (int yy = 0; yy < m->height; yy++) {
+        const int r = source_row(*m, yy);
+        if (r == INT_MIN)
             return set_error(TOPS_ERR_ARG,
                              "%s: output row %d (line time %.9f) lies in no burst", who, yy,
                              m->t_first + yy * m->dt);
+        if (rows) (*rows)[yy] = r;
+    }
     return TOPS_OK;
+}
+
+// ---- device gather ---------------------------------------------------------
+constexpr int kThreads = 256;
+constexpr int kMaxRuns = 64;        // runs per launch (kernel argument space)
+
+// output rows start[k] .. start[k+1]-1 copy input rows yy + offset[k]
+struct RowRuns {
+    int n;                          // runs in this launch
+    int y_begin, y_end;             // output rows of this launch
+    int gapped_rows;
+    int start[kMaxRuns];
+    int offset[kMaxRuns];
+};
+
+typedef unsigned int Vec16 __attribute__((ext_vector_type(4)));
+
+template <typename V>
+__global__ __launch_bounds__(kThreads) void deburst_kernel(RowRuns runs, const V* src, V* dst,
+                                                           long row_vecs, int chunks)
+{
+    const int yy = runs.y_begin + (int)(blockIdx.x / (unsigned)chunks);
+    const int chunk = (int)(blockIdx.x % (unsigned)chunks);
+    int k = 0;                                    // the same for the whole workgroup
+    while (k + 1 < runs.n && yy >= runs.start[k + 1]) k++;
+    const int row = yy + runs.offset[k];
+    const long i = (long)chunk * kThreads + threadIdx.x;
+    if (i >= row_vecs) return;
+    V* out = dst + (size_t)yy * row_vecs + i;
+    if (row >= 0 && row < runs.gapped_rows) {
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + (size_t)row * row_vecs + i),
+                                    out);
+    } else {
+        // the reference's read of a row outside the gapped raster fails and
+        // leaves the zeroed line buffer (Func.cpp:2880-2883)
+        const V z = {};
+        __builtin_nontemporal_store(z, out);
+    }
+}
+
+int launch_gather(const std::vector<int>& rows, int gapped_rows, const void* src, void* dst,
+                  size_t row_bytes, hipStream_t s)
+{
+    const int height = (int)rows.size();
+    if (height == 0 || row_bytes == 0) return TOPS_OK;
+    const bool v16 = row_bytes % 16 == 0 && (uintptr_t)src % 16 == 0 && (uintptr_t)dst % 16 == 0;
+    if (!v16 && (row_bytes % 4 != 0 || (uintptr_t)src % 4 != 0 || (uintptr_t)dst % 4 != 0))
+        return set_error(TOPS_ERR_ARG, "deburst: rasters must be 4-byte aligned");
+    const size_t row_vecs = row_bytes / (v16 ? 16 : 4);
+    const size_t chunks = (row_vecs + kThreads - 1) / kThreads;
+    if (chunks * (size_t)height > (size_t)INT_MAX)
+        return set_error(TOPS_ERR_ARG, "deburst: output too large");
+    // runs of constant offset, at most kMaxRuns per launch
+    int y = 0;
+    while (y < height) {
+        RowRuns r;
+        memset(&r, 0, sizeof(r));
+        r.gapped_rows = gapped_rows;
+        r.y_begin = y;
+        while (y < height) {
+            const int off = rows[y] - y;
+            if (r.n == 0 || off != r.offset[r.n - 1]) {
+                if (r.n == kMaxRuns) break;
+                r.start[r.n] = y;
+                r.offset[r.n] = off;
+                r.n++;
+            }
+            y++;
+        }
+        r.y_end = y;
+        const unsigned blocks = (unsigned)(chunks * (size_t)(r.y_end - r.y_begin));
+        if (v16)
+            deburst_kernel<Vec16><<<dim3(blocks), dim3(kThreads), 0, s>>>(
+                r, (const Vec16*)src, (Vec16*)dst, (long)row_vecs, (int)chunks);
+        else
+            deburst_kernel<unsigned int><<<dim3(blocks), dim3(kThreads), 0, s>>>(
+                r, (const unsigned int*)src, (unsigned int*)dst, (long)row_vecs, (int)chunks);
+        TOPS_HIP(hipGetLastError());
+    }
+    return TOPS_OK;
+}
+
+int element_bytes(int data_type)
+{
+    switch (data_type) {
+    case TOPS_GDT_CINT16: return 4;
+    case TOPS_GDT_FLOAT32: return 4;
+    case TOPS_GDT_CFLOAT32: return 8;
+    default: return 0;
+    }
 }
 
 struct DevMem {
@@ -158,8 +238,8 @@ extern "C" {
 int tops_deburst_extent(const tops_deburst_swath* sw, int burst0, int burstN, int* y_min,
                         int* height)
 {
-    DeburstMap m;
-    const int rc = make_map(sw, burst0, burstN, &m, y_min, "tops_deburst_extent");
+    LineMap m;
+    const int rc = make_map(sw, burst0, burstN, &m, y_min, nullptr, "tops_deburst_extent");
     if (rc) return rc;
     if (height) *height = m.height;
     return TOPS_OK;
@@ -167,11 +247,12 @@ int tops_deburst_extent(const tops_deburst_swath* sw, int burst0, int burstN, in
 
 int tops_deburst_rows(const tops_deburst_swath* sw, int burst0, int burstN, int* rows)
 {
-    DeburstMap m;
-    const int rc = make_map(sw, burst0, burstN, &m, nullptr, "tops_deburst_rows");
+    LineMap m;
+    std::vector<int> r;
+    const int rc = make_map(sw, burst0, burstN, &m, nullptr, &r, "tops_deburst_rows");
     if (rc) return rc;
     if (!rows) return set_error(TOPS_ERR_ARG, "tops_deburst_rows: null rows");
-    for (int yy = 0; yy < m.height; yy++) rows[yy] = deburst_source_row(m, yy);
+    memcpy(rows, r.data(), sizeof(int) * r.size());
     return TOPS_OK;
 }
 
@@ -183,12 +264,13 @@ int tops_deburst(const tops_deburst_swath* sw, int burst0, int burstN, int data_
     if (!eb)
         return set_error(TOPS_ERR_ARG, "%s: unsupported image type %d for the stitching step",
                          who, data_type);
-    DeburstMap m;
-    int rc = make_map(sw, burst0, burstN, &m, nullptr, who);
+    LineMap m;
+    std::vector<int> rows;
+    int rc = make_map(sw, burst0, burstN, &m, nullptr, &rows, who);
     if (rc) return rc;
     if (!d_gapped || !d_out) return set_error(TOPS_ERR_ARG, "%s: null raster", who);
-    return launch_deburst(m, d_gapped, d_out, (size_t)sw->samples_per_burst * eb,
-                          (hipStream_t)stream);
+    return launch_gather(rows, m.gapped_rows, d_gapped, d_out, (size_t)sw->samples_per_burst * eb,
+                         (hipStream_t)stream);
 }
 
 int tops_stitch_subswath(const tops_deburst_swath* sw, int burst0, int burstN, int data_type,
@@ -199,8 +281,9 @@ int tops_stitch_subswath(const tops_deburst_swath* sw, int burst0, int burstN, i
     if (!eb)
         return set_error(TOPS_ERR_ARG, "%s: unsupported image type %d for the stitching step",
                          who, data_type);
-    DeburstMap m;
-    int rc = make_map(sw, burst0, burstN, &m, nullptr, who);
+    LineMap m;
+    std::vector<int> rows;
+    int rc = make_map(sw, burst0, burstN, &m, nullptr, &rows, who);
     if (rc) return rc;
     if (!gapped || !out) return set_error(TOPS_ERR_ARG, "%s: null raster", who);
     const size_t row_bytes = (size_t)sw->samples_per_burst * eb;
@@ -209,7 +292,7 @@ int tops_stitch_subswath(const tops_deburst_swath* sw, int burst0, int burstN, i
     TOPS_HIP(hipMalloc(&din.p, in_bytes));
     TOPS_HIP(hipMalloc(&dout.p, out_bytes));
     TOPS_HIP(hipMemcpy(din.p, gapped, in_bytes, hipMemcpyHostToDevice));
-    rc = launch_deburst(m, din.p, dout.p, row_bytes, nullptr);
+    rc = launch_gather(rows, m.gapped_rows, din.p, dout.p, row_bytes, nullptr);
     if (rc) return rc;
     TOPS_HIP(hipMemcpy(out, dout.p, out_bytes, hipMemcpyDeviceToHost));
     return TOPS_OK;

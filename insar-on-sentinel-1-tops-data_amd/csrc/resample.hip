@@ -33,6 +33,7 @@
 // With y_j = y0 + (j - 5) and Mk = sum_j kL_j (j-5)^k (float LUT values in
 // FP64), sum_j kL_j phi(y_j, x) = M0 a + (y0 M0 + M1) b + (y0^2 M0 + 2 y0 M1
 // + M2) c exactly, so Phi needs three 12-tap FP64 range dot products.
+#include "stream_access.hpp"
 #include "tops_internal.hpp"
 
 #include <algorithm>
@@ -553,7 +554,8 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
             o = interior_direct(p.slave, p.kt, p.tref, p.fdc, p.mom, p.kaz, p.krg, p.s_pixels,
                                 p.sb[0], p.sb[2], p.first_line, p.dt, iy, fy, ix, fx);
         }
-        if (out_lane) p.out[(size_t)r * p.m_pixels + c] = o;
+        // written once: a nontemporal store
+        if (out_lane) stream_store_float2(p.out + (size_t)r * p.m_pixels + c, o.x, o.y);
     }
 }
 

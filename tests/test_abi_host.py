@@ -123,3 +123,16 @@ def test_read_config(tops, tmp_path):
     assert (cfg.firstsubswath, cfg.lastsubswath, cfg.burst0, cfg.burstN) == (1, 3, 2, 7)
     assert N.load().tops_read_config(b"/nonexistent/config.txt", C.byref(cfg)) == N.TOPS_ERR_ARG
     N.load().tops_clear_error()
+
+
+def test_reference_entry_points_clear_the_status(tops):
+    """The void reference entry points report failure only through the
+    status: each clears it on entry, so a check after a successful call does
+    not see an earlier call's error.  (EstESDShifts with no overlaps returns
+    before any device work.)"""
+    N = tops.native
+    L = N.load()
+    assert L.tops_read_config(b"/nonexistent/config.txt", C.byref(N.Config())) == N.TOPS_ERR_ARG
+    assert L.tops_last_error() == N.TOPS_ERR_ARG
+    L.EstESDShifts(None, None, None, None, None, None, 0, 0, 0, 6, 24, 0, 0.3, 1.0, 1.0)
+    assert L.tops_last_error() == N.TOPS_OK

@@ -98,6 +98,27 @@ def test_deburst_narrow_rows_and_zero_lines(tops, orc, dev):
     assert not got[rows < 0].any()
 
 
+def test_deburst_many_runs(tops, orc, dev):
+    """A line map of 263 runs of constant source offset (the rounding of
+    half-line burst offsets): the gather spans several launches."""
+    from test_oracle_deburst import wobble_swath
+    N = tops.native
+    c = wobble_swath()
+    arrs = [np.ascontiguousarray(c[k], dtype=np.float64)
+            for k in ("first", "last", "first_valid", "last_valid")]
+    width = 1000
+    sw = N.DeburstSwath(c["lpb"], width, 4, c["num_lines"], c["dt"], c["first_line_time"],
+                        *[N.ptr(a) for a in arrs])
+    rng = np.random.default_rng(11)
+    g = (rng.standard_normal((4 * c["lpb"], width)) +
+         1j * rng.standard_normal((4 * c["lpb"], width))).astype(np.complex64)
+    _, _, got = _device_deburst(tops, dev, sw, 0, 3, g)
+    ref, rows = _oracle(orc, arrs, c["lpb"], c["num_lines"], c["dt"], c["first_line_time"], 0, 3,
+                        g)
+    assert 1 + np.count_nonzero(np.diff(rows - np.arange(len(rows)))) > 64
+    assert np.array_equal(got, ref)
+
+
 def test_stitch_subswath_host_buffers(tops, orc, run):
     """tops_stitch_subswath (host rasters in and out) gives the oracle's
     stitch of the master SLC."""
@@ -117,6 +138,7 @@ def test_deburst_errors(tops, dev, run):
     assert "unsupported image type" in N.last_error()[1]
     assert L.tops_deburst(C.byref(sw), 0, 3, N.GDT_FLOAT32, d.data_ptr(), d.data_ptr(),
                           None) == N.TOPS_ERR_ARG
+    L.tops_clear_error()
 
 
 def test_deburst_graph_replay(tops, dev, run):

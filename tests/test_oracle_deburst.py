@@ -169,3 +169,25 @@ def test_library_deburst_rows_match_oracle(tops, orc, iw, b0, bn):
                                 c["dt"])
     assert y0 == oy0
     assert np.array_equal(rows, _oracle_rows(orc, c))
+
+
+def wobble_swath():
+    """4 bursts of 400 lines, dt = 0.3 s, odd bursts offset by half a line:
+    (t - first) / dt sits on .5 in those bursts and the rounding of
+    getLineIdxInMetaData alternates between neighbouring lines (263 runs of
+    constant source offset)."""
+    dt, lpb, nb = 0.3, 400, 4
+    first = np.array([43200.0 + i * 300 * dt + (0.15 if i % 2 else 0.0) for i in range(nb)])
+    return dict(first=first, last=first + (lpb - 1) * dt, first_valid=first + 2 * dt,
+                last_valid=first + (lpb - 3) * dt, lpb=lpb, num_lines=nb * lpb, dt=dt,
+                first_line_time=float(first[0]), b0=0, bn=nb - 1)
+
+
+def test_library_deburst_rows_wobble(tops, orc):
+    c = wobble_swath()
+    rc, _, rows = _lib_rows(tops, c)
+    assert rc == 0
+    ref = _oracle_rows(orc, c)
+    assert np.array_equal(rows, ref)
+    runs = 1 + np.count_nonzero(np.diff(ref - np.arange(len(ref))))
+    assert runs > 64
