@@ -143,3 +143,33 @@ def test_plan_tune_and_graph_replay(tops, dev, batch):
             assert np.array_equal(dev.get(d["out"], np.complex64, (b["L"], b["P"])), eo), rep
             assert np.array_equal(dev.get(d["coh"], np.float32, (b["L"], b["P"])), ec), rep
     L.tops_plan_destroy(plan)
+
+
+def test_plan_per_burst_graph_replay(tops, dev, batch):
+    """The per-burst schedule (a resample and a coherence launch per burst)
+    captured into a hipGraph and replayed gives the eager result."""
+    N = tops.native
+    L = N.load()
+    torch = dev.torch
+    plan = L.tops_plan_create(batch["ctx"], batch["items"], 5, 3, 10, 0)
+    assert plan, N.last_error()
+    s = torch.cuda.Stream()
+    sp = C.c_void_p(s.cuda_stream)
+    N.check(L.tops_plan_run(plan, sp))
+    s.synchronize()
+    eager = [(dev.get(d["out"], np.complex64, (b["L"], b["P"])).copy(),
+              dev.get(d["coh"], np.float32, (b["L"], b["P"])).copy())
+             for b, d in zip(batch["bursts"], batch["dev"])]
+    print("  plan: per-burst schedule graph capture", flush=True)
+    N.check(L.tops_plan_capture_graph(plan, sp), "tops_plan_capture_graph")
+    for rep in range(3):
+        for d in batch["dev"]:
+            d["out"].zero_()
+            d["coh"].zero_()
+        torch.cuda.synchronize()
+        N.check(L.tops_plan_run(plan, sp))
+        s.synchronize()
+        for (b, d), (eo, ec) in zip(zip(batch["bursts"], batch["dev"]), eager):
+            assert np.array_equal(dev.get(d["out"], np.complex64, (b["L"], b["P"])), eo), rep
+            assert np.array_equal(dev.get(d["coh"], np.float32, (b["L"], b["P"])), ec), rep
+    L.tops_plan_destroy(plan)
