@@ -428,6 +428,96 @@ int tops_read_config(const char* path, tops_config* cfg);
 int tops_stitch_subswath(const tops_deburst_swath* sw, int burst0, int burstN, int data_type,
                          const void* gapped, void* out);
 
+/* ======================================================================== */
+/* Part 4: ingest without GDAL or tinyxml2                                  */
+/* ======================================================================== */
+
+/* Replaces utc2seconds, Func.cpp:629-635: seconds of the day of an
+ * "YYYY-MM-DDTHH:MM:SS.ffffff" time (the date is ignored, as there). */
+double tops_utc2seconds(const char* utc);
+
+/* One subswath's annotation XML: the fields S1_Initialize reads
+ * (Func.cpp:205-556) and the per-burst Doppler centroid and FM-rate
+ * polynomials of ComputeDerampPara (Func.cpp:1868-2031, computeDC
+ * 1828-1866).  Times are seconds of the day; slant-range times one way.
+ * Arrays are allocated by tops_annotation_read and released by
+ * tops_annotation_free. */
+typedef struct tops_annotation {
+    char polarisation[8];
+    int num_lines, num_samples;
+    double first_line_time, last_line_time;
+    double az_time_interval, prf;
+    double range_pixel_spacing, azimuth_pixel_spacing;
+    double radar_frequency, wavelength, range_sampling_rate;
+    double azimuth_steering_rate, heading, ascending_node_time;
+    double azimuth_bandwidth, range_bandwidth;
+    double slr_time_first_pixel, slr_time_last_pixel;
+    int n_bursts, lines_per_burst, samples_per_burst;
+    double* burst_first_line_time;        /* [n_bursts] */
+    double* burst_last_line_time;         /* [n_bursts] */
+    double* burst_first_valid_line_time;  /* [n_bursts] */
+    double* burst_last_valid_line_time;   /* [n_bursts] */
+    int* first_valid_line;                /* [n_bursts] */
+    int* last_valid_line;                 /* [n_bursts] */
+    int* first_valid_sample;              /* [n_bursts * lines_per_burst] */
+    int* last_valid_sample;               /* [n_bursts * lines_per_burst] */
+    int first_valid_pixel, last_valid_pixel;
+    double slr_time_first_valid_pixel, slr_time_last_valid_pixel;
+    int geo_lines, geo_points_per_line;   /* geolocation grid */
+    double* geo_az_time;                  /* [geo_lines * geo_points_per_line] */
+    double* geo_slant_range_time;
+    double* geo_latitude;
+    double* geo_longitude;
+    double* geo_incidence_angle;
+    int near_range_on_left;
+    double* burst_ll;                     /* [n_bursts][4] lat_min lat_max lon_min lon_max */
+    double* dc;                           /* [n_bursts][4] t0 c0 c1 c2 (tops_deramp_params) */
+    double* fm;                           /* [n_bursts][4] t0 c0 c1 c2 */
+} tops_annotation;
+int tops_annotation_read(const char* xml_path, tops_annotation* a);
+void tops_annotation_free(tops_annotation* a);
+
+/* Precise orbit (EOF): the state vectors S1_OrbitInitialize takes
+ * (Func.cpp:704-819): from 4 vectors before the one in the 10 s slot of
+ * first_utc, tt/10 + 10 vectors (tt = whole seconds from first_utc to
+ * last_utc).  Feed t, x, y, z to tops_polyfit_orbit. */
+typedef struct tops_eof_orbit {
+    int n;
+    double* t;                            /* seconds of the day */
+    double *x, *y, *z, *vx, *vy, *vz;
+    double time_interval;
+} tops_eof_orbit;
+int tops_eof_read(const char* eof_path, const char* first_utc, const char* last_utc,
+                  tops_eof_orbit* o);
+void tops_eof_free(tops_eof_orbit* o);
+
+/* Stripped, uncompressed, single-band TIFF / BigTIFF (TiffRead / TiffWrite,
+ * DataTypes.cpp:417-620) of TOPS_GDT_* types CInt16, Float32, CFloat32 and
+ * 7 (Float64).  Windows are (x0, y0, lines, pixels) as GDALRasterIO takes
+ * them; a window outside the raster is an error and leaves the buffer
+ * untouched. */
+typedef struct tops_tiff tops_tiff;
+#define TOPS_GDT_FLOAT64 7
+tops_tiff* tops_tiff_open(const char* path);
+int tops_tiff_info(const tops_tiff* t, int* pixels, int* lines, int* data_type);
+int tops_tiff_read(tops_tiff* t, int x0, int y0, int lines, int pixels, int data_type,
+                   void* buffer);
+/* The overlap-band read of DataTypes.cpp:456-513: for overlap i, lines
+ * y0+(i+1)*lpb-n_i .. (forward) and y0+(i+1)*lpb .. (backward), n_i lines
+ * each, concatenated. */
+int tops_tiff_read_overlaps(tops_tiff* t, void* data_for, void* data_back, int n_overlap,
+                            const int* overlap_size, int lines_per_burst, int x0, int y0,
+                            int width, int data_type);
+/* The window straight into device memory: file reads into pinned staging
+ * buffers overlapped with their copies to d_buffer on `stream`; returns
+ * after the last copy has completed. */
+int tops_tiff_read_device(tops_tiff* t, int x0, int y0, int lines, int pixels, int data_type,
+                          void* d_buffer, void* stream);
+tops_tiff* tops_tiff_create(const char* path, int data_type, int pixels, int lines);
+int tops_tiff_write(tops_tiff* t, int x0, int y0, int lines, int pixels, int data_type,
+                    const void* buffer);
+int tops_tiff_close(tops_tiff* t);
+
 #ifdef __cplusplus
 }
 #endif

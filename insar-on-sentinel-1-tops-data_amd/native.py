@@ -20,6 +20,7 @@ MODE_FAST = 0
 MODE_FP64 = 1
 GDT_FLOAT32 = 6
 GDT_CINT16 = 8
+GDT_FLOAT64 = 7
 GDT_CFLOAT32 = 10
 
 # every symbol declared in include/tops_abi.h
@@ -36,6 +37,10 @@ EXPORTS = [
     "tops_plan_strip_rows", "tops_plan_destroy",
     "tops_subswath_workspace_size", "tops_subswath_esd_coh",
     "tops_deburst_extent", "tops_deburst_rows", "tops_deburst", "tops_stitch_subswath",
+    "tops_utc2seconds", "tops_annotation_read", "tops_annotation_free", "tops_eof_read",
+    "tops_eof_free", "tops_tiff_open", "tops_tiff_info", "tops_tiff_read",
+    "tops_tiff_read_overlaps", "tops_tiff_read_device", "tops_tiff_create", "tops_tiff_write",
+    "tops_tiff_close",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -80,6 +85,36 @@ class DeburstSwath(C.Structure):
                 ("first_line_time", C.c_double), ("burst_first_line_time", _dp),
                 ("burst_last_line_time", _dp), ("burst_first_valid_line_time", _dp),
                 ("burst_last_valid_line_time", _dp)]
+
+
+class Annotation(C.Structure):
+    _fields_ = [("polarisation", C.c_char * 8), ("num_lines", C.c_int), ("num_samples", C.c_int),
+                ("first_line_time", C.c_double), ("last_line_time", C.c_double),
+                ("az_time_interval", C.c_double), ("prf", C.c_double),
+                ("range_pixel_spacing", C.c_double), ("azimuth_pixel_spacing", C.c_double),
+                ("radar_frequency", C.c_double), ("wavelength", C.c_double),
+                ("range_sampling_rate", C.c_double), ("azimuth_steering_rate", C.c_double),
+                ("heading", C.c_double), ("ascending_node_time", C.c_double),
+                ("azimuth_bandwidth", C.c_double), ("range_bandwidth", C.c_double),
+                ("slr_time_first_pixel", C.c_double), ("slr_time_last_pixel", C.c_double),
+                ("n_bursts", C.c_int), ("lines_per_burst", C.c_int),
+                ("samples_per_burst", C.c_int),
+                ("burst_first_line_time", _dp), ("burst_last_line_time", _dp),
+                ("burst_first_valid_line_time", _dp), ("burst_last_valid_line_time", _dp),
+                ("first_valid_line", _ip), ("last_valid_line", _ip),
+                ("first_valid_sample", _ip), ("last_valid_sample", _ip),
+                ("first_valid_pixel", C.c_int), ("last_valid_pixel", C.c_int),
+                ("slr_time_first_valid_pixel", C.c_double),
+                ("slr_time_last_valid_pixel", C.c_double),
+                ("geo_lines", C.c_int), ("geo_points_per_line", C.c_int),
+                ("geo_az_time", _dp), ("geo_slant_range_time", _dp), ("geo_latitude", _dp),
+                ("geo_longitude", _dp), ("geo_incidence_angle", _dp),
+                ("near_range_on_left", C.c_int), ("burst_ll", _dp), ("dc", _dp), ("fm", _dp)]
+
+
+class EofOrbit(C.Structure):
+    _fields_ = [("n", C.c_int), ("t", _dp), ("x", _dp), ("y", _dp), ("z", _dp), ("vx", _dp),
+                ("vy", _dp), ("vz", _dp), ("time_interval", C.c_double)]
 
 
 class Swath(C.Structure):
@@ -228,6 +263,34 @@ def load():
     L.tops_deburst.argtypes = [_dsw, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]
     L.tops_stitch_subswath.restype = C.c_int
     L.tops_stitch_subswath.argtypes = [_dsw, C.c_int, C.c_int, C.c_int, _vp, _vp]
+    L.tops_utc2seconds.restype = C.c_double
+    L.tops_utc2seconds.argtypes = [C.c_char_p]
+    L.tops_annotation_read.restype = C.c_int
+    L.tops_annotation_read.argtypes = [C.c_char_p, C.POINTER(Annotation)]
+    L.tops_annotation_free.restype = None
+    L.tops_annotation_free.argtypes = [C.POINTER(Annotation)]
+    L.tops_eof_read.restype = C.c_int
+    L.tops_eof_read.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(EofOrbit)]
+    L.tops_eof_free.restype = None
+    L.tops_eof_free.argtypes = [C.POINTER(EofOrbit)]
+    L.tops_tiff_open.restype = _vp
+    L.tops_tiff_open.argtypes = [C.c_char_p]
+    L.tops_tiff_info.restype = C.c_int
+    L.tops_tiff_info.argtypes = [_vp, _ip, _ip, _ip]
+    L.tops_tiff_read.restype = C.c_int
+    L.tops_tiff_read.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]
+    L.tops_tiff_read_overlaps.restype = C.c_int
+    L.tops_tiff_read_overlaps.argtypes = [_vp, _vp, _vp, C.c_int, _ip, C.c_int, C.c_int,
+                                          C.c_int, C.c_int, C.c_int]
+    L.tops_tiff_read_device.restype = C.c_int
+    L.tops_tiff_read_device.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp,
+                                        _vp]
+    L.tops_tiff_create.restype = _vp
+    L.tops_tiff_create.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int]
+    L.tops_tiff_write.restype = C.c_int
+    L.tops_tiff_write.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]
+    L.tops_tiff_close.restype = C.c_int
+    L.tops_tiff_close.argtypes = [_vp]
     L.tops_read_config.restype = C.c_int
     L.tops_read_config.argtypes = [C.c_char_p, C.POINTER(Config)]
     _lib = L
