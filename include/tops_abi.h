@@ -518,6 +518,24 @@ int tops_tiff_write(tops_tiff* t, int x0, int y0, int lines, int pixels, int dat
                     const void* buffer);
 int tops_tiff_close(tops_tiff* t);
 
+/* Burst matching for pairs whose burst grids are offset.  Replaces
+ * xyz2aztime (Func.cpp:1349-1377): zero-Doppler time of xyz on a fitted
+ * orbit, Newton from t_init. */
+double tops_xyz2aztime(const double* xyz, const tops_orbit* orbit, double t_init);
+/* Replaces lp2xyz (Func.cpp:1492-1584): ground position (WGS84 ellipsoid)
+ * of 1-based (line, pixel) of burst `burst`, Newton from the position in
+ * xyz; returns the iteration count (-1 on bad arguments). */
+long tops_lp2xyz(double line, double pixel, const tops_annotation* a, const tops_orbit* orbit,
+                 int burst, double* xyz, int max_iter, double criter);
+/* Replaces ComputerBurstOverlap (Func.cpp:1610-1746): burst0 / burstN are
+ * the 1-based master bursts asked for (config.txt); outputs are the 0-based
+ * matching master and slave burst ranges.  TOPS_ERR_ARG when the pair does
+ * not overlap. */
+int tops_burst_overlap(const tops_annotation* master, const tops_annotation* slave,
+                       const tops_orbit* master_orbit, const tops_orbit* slave_orbit,
+                       int burst0, int burstN, int* m_burst0, int* m_burstN, int* s_burst0,
+                       int* s_burstN);
+
 #ifdef __cplusplus
 }
 #endif

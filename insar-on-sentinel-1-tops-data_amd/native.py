@@ -40,7 +40,7 @@ EXPORTS = [
     "tops_utc2seconds", "tops_annotation_read", "tops_annotation_free", "tops_eof_read",
     "tops_eof_free", "tops_tiff_open", "tops_tiff_info", "tops_tiff_read",
     "tops_tiff_read_overlaps", "tops_tiff_read_device", "tops_tiff_create", "tops_tiff_write",
-    "tops_tiff_close",
+    "tops_tiff_close", "tops_xyz2aztime", "tops_lp2xyz", "tops_burst_overlap",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -291,6 +291,15 @@ def load():
     L.tops_tiff_write.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]
     L.tops_tiff_close.restype = C.c_int
     L.tops_tiff_close.argtypes = [_vp]
+    _op = C.POINTER(Orbit)
+    _ap = C.POINTER(Annotation)
+    L.tops_xyz2aztime.restype = C.c_double
+    L.tops_xyz2aztime.argtypes = [_dp, _op, C.c_double]
+    L.tops_lp2xyz.restype = C.c_long
+    L.tops_lp2xyz.argtypes = [C.c_double, C.c_double, _ap, _op, C.c_int, _dp, C.c_int,
+                              C.c_double]
+    L.tops_burst_overlap.restype = C.c_int
+    L.tops_burst_overlap.argtypes = [_ap, _ap, _op, _op, C.c_int, C.c_int, _ip, _ip, _ip, _ip]
     L.tops_read_config.restype = C.c_int
     L.tops_read_config.argtypes = [C.c_char_p, C.POINTER(Config)]
     _lib = L

@@ -348,11 +348,13 @@ def _swath(name_seed, lines, pixels, n_bursts, t_first, cycle_lines, slr0, dc_sl
 def make_pair(name, lines, pixels, n_bursts=1, overlap=100, seed=20260101, stream=0,
               az_off=BILINEAR_AZ, rg_off=BILINEAR_RG, delta=0.0, phase_noise=0.2,
               dem_spacing=3.0, dc_slope_total=5e-3 * 2047, scene_center=(30.0, 100.0),
-              slr_shift=0.0, slave_dc0=20.0):
+              slr_shift=0.0, slave_dc0=20.0, slave_burst_offset=0):
     """Burst-aligned master/slave pair; the slave orbit carries a ~120 m
     baseline and a small along-track delay so geocoreg sees real offsets.
     slave_dc0: the slave's Doppler centroid at the first range sample (Hz);
-    the master's is 20 Hz."""
+    the master's is 20 Hz.  slave_burst_offset: the slave's bursts start
+    that many burst cycles after the master's (metadata only: the burst
+    rasters assume aligned bursts)."""
     tc = 43200.0
     morbit = make_orbit(scene_center[0], scene_center[1], tc)
     cycle = lines - overlap if n_bursts > 1 else lines
@@ -366,8 +368,9 @@ def make_pair(name, lines, pixels, n_bursts=1, overlap=100, seed=20260101, strea
     up = pc / np.linalg.norm(pc)
     sorbit = make_orbit(scene_center[0], scene_center[1], tc + 0.2 * AZ_TIME_INTERVAL,
                         offset=80.0 * morbit.e1 + 90.0 * np.cross(morbit.e2, morbit.e1))
-    slave = _swath(seed + 1, lines, pixels, n_bursts, t_first, cycle, slr0, dc_slope_total, sorbit,
-                   scene_center, dc0=slave_dc0)
+    slave = _swath(seed + 1, lines, pixels, n_bursts,
+                   t_first + slave_burst_offset * cycle * AZ_TIME_INTERVAL, cycle, slr0,
+                   dc_slope_total, sorbit, scene_center, dc0=slave_dc0)
     del up
     return Pair(name=name, seed=seed, stream=stream, master=master, slave=slave, morbit=morbit,
                 sorbit=sorbit, az_off=np.asarray(az_off, float), rg_off=np.asarray(rg_off, float),

@@ -34,6 +34,20 @@ class Poly(C.Structure):
     _fields_ = [("t0", C.c_double), ("c0", C.c_double), ("c1", C.c_double), ("c2", C.c_double)]
 
 
+class MatchSwath(C.Structure):
+    _fields_ = [("n_bursts", C.c_int), ("lpb", C.c_int), ("spb", C.c_int), ("dt", C.c_double),
+                ("prf", C.c_double), ("rsr", C.c_double), ("slr_first", C.c_double),
+                ("slr_last", C.c_double), ("first_line_time", C.c_double),
+                ("last_line_time", C.c_double), ("burst_first_line_time", _dp),
+                ("geo_lines", C.c_int), ("geo_ppl", C.c_int), ("geo_az", _dp), ("geo_rg", _dp),
+                ("geo_lat", _dp), ("geo_lon", _dp)]
+
+
+class MatchOrbit(C.Structure):
+    _fields_ = [("num_points", C.c_int), ("t", _dp), ("ncoef", C.c_int), ("cx", _dp),
+                ("cy", _dp), ("cz", _dp)]
+
+
 _lib = None
 
 
@@ -80,6 +94,15 @@ def load():
                               C.c_double, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
                               C.POINTER(C.c_int)]
     L.orc_deburst.restype = C.c_int
+    L.orc_xyz2aztime.restype = C.c_double
+    L.orc_xyz2aztime.argtypes = [_dp, C.POINTER(MatchOrbit), C.c_double]
+    L.orc_lp2xyz.restype = C.c_int
+    L.orc_lp2xyz.argtypes = [C.c_double, C.c_double, C.POINTER(MatchSwath),
+                             C.POINTER(MatchOrbit), C.c_int, _dp]
+    L.orc_burst_overlap.restype = C.c_int
+    L.orc_burst_overlap.argtypes = [C.POINTER(MatchSwath), C.POINTER(MatchSwath),
+                                    C.POINTER(MatchOrbit), C.POINTER(MatchOrbit), C.c_int,
+                                    C.c_int, C.POINTER(C.c_int)]
     _lib = L
     return L
 

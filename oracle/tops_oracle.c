@@ -670,3 +670,203 @@ int orc_deburst(const double* first_line_time, const double* last_line_time,
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Burst matching: Locating / getLatitude (Func.cpp:1133-1246),             */
+/* Geodetic2Geographic (1248-1260), orbit polynomial (1262-1345, 1409-1453),*/
+/* xyz2aztime (1349-1377), solve33 (1462-1490), lp2xyz (1492-1584),         */
+/* ComputerBurstOverlap (1610-1746)                                         */
+/* ------------------------------------------------------------------------ */
+static double orc_dot3(const double* a, const double* b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* position / velocity / acceleration at absolute time t, polynomial in
+ * (t - mid) / 10 with mid = t[num_points / 2] */
+static void orc_match_orbit_eval(const orc_match_orbit* o, double t, double* p, double* v,
+                                 double* acc)
+{
+    const double tn = (t - o->t[o->num_points / 2]) / 10.0;
+    const double* c[3] = {o->cx, o->cy, o->cz};
+    for (int k = 0; k < 3; k++) {
+        double s = 0.0;
+        for (int i = o->ncoef - 1; i >= 0; i--) s = s * tn + c[k][i];
+        if (p) p[k] = s;
+        if (v) {
+            double d = c[k][1];
+            for (int i = 2; i < o->ncoef; i++) d += c[k][i] * ((double)i * pow(tn, i - 1));
+            v[k] = d / 10.0;
+        }
+        if (acc) {
+            double d = 0.0;
+            for (int i = 2; i < o->ncoef; i++) d += c[k][i] * ((double)((i - 1) * i) * pow(tn, i - 2));
+            acc[k] = d / 100.0;
+        }
+    }
+}
+
+double orc_xyz2aztime(const double* xyz, const orc_match_orbit* o, double t_init)
+{
+    double t = t_init;
+    for (int it = 0; it <= 10; it++) {
+        double p[3], v[3], a[3], d[3];
+        orc_match_orbit_eval(o, t, p, v, a);
+        for (int k = 0; k < 3; k++) d[k] = xyz[k] - p[k];
+        const double sol = -orc_dot3(v, d) / (orc_dot3(a, d) - orc_dot3(v, v));
+        t += sol;
+        if (fabs(sol) < 1e-10) break;
+    }
+    return t;
+}
+
+/* Locating + bilinear interpolation of the grid's latitude / longitude
+ * (the interpolation weights kept fractional) */
+static void orc_grid_ll(const orc_match_swath* s, double az, double rg, double* lat, double* lon)
+{
+    const int P = s->geo_ppl, G = s->geo_lines;
+    int j0 = -1, j1 = -1;
+    if (rg < s->geo_rg[0]) {
+        j0 = 0;
+        j1 = 1;
+    } else if (rg > s->geo_rg[P - 1]) {
+        j0 = P - 2;
+        j1 = P - 1;
+    } else {
+        for (int j = 0; j < P - 1; j++)
+            if (s->geo_rg[j] <= rg && s->geo_rg[j + 1] > rg) {
+                j0 = j;
+                j1 = j + 1;
+                break;
+            }
+    }
+    if (j0 < 0) { j0 = P - 2; j1 = P - 1; }
+    const double mx = (rg - s->geo_rg[j0]) / (s->geo_rg[j1] - s->geo_rg[j0]);
+    int i0 = -1;
+    double my = 0.0;
+    for (int i = 0; i < G - 1; i++) {
+        const double a0 = (1 - mx) * s->geo_az[i * P + j0] + mx * s->geo_az[i * P + j1];
+        const double a1 = (1 - mx) * s->geo_az[(i + 1) * P + j0] + mx * s->geo_az[(i + 1) * P + j1];
+        if ((i == 0 && az < a0) || (i == G - 2 && az >= a1) || (a0 <= az && a1 > az)) {
+            i0 = i;
+            my = (az - a0) / (a1 - a0);
+            break;
+        }
+    }
+    if (i0 < 0) i0 = 0;
+    const int i1 = i0 + 1;
+    const double* g[2] = {s->geo_lat, s->geo_lon};
+    double out[2];
+    for (int k = 0; k < 2; k++) {
+        const double v00 = g[k][i0 * P + j0], v01 = g[k][i0 * P + j1];
+        const double v10 = g[k][i1 * P + j0], v11 = g[k][i1 * P + j1];
+        out[k] = (1 - my) * ((1 - mx) * v00 + mx * v01) + my * ((1 - mx) * v10 + mx * v11);
+    }
+    *lat = out[0];
+    *lon = out[1];
+}
+
+static void orc_llh2xyz_wgs84(double lat, double lon, double h, double* xyz)
+{
+    const double a = ORC_ELL_A, b = ORC_ELL_B, e2 = 1.0 - (b / a) * (b / a);
+    const double N = a / sqrt(1.0 - e2 * (sin(lat) * sin(lat)));
+    const double Nph = N + h;
+    xyz[0] = Nph * cos(lat) * cos(lon);
+    xyz[1] = Nph * cos(lat) * sin(lon);
+    xyz[2] = (Nph - e2 * N) * sin(lat);
+}
+
+static void orc_solve33(double* x, const double* rhs, const double* A)
+{
+    const double L10 = A[3] / A[0], L20 = A[6] / A[0];
+    const double U11 = A[4] - L10 * A[1];
+    const double L21 = (A[7] - A[1] * L20) / U11;
+    const double U12 = A[5] - L10 * A[2];
+    const double U22 = A[8] - L20 * A[2] - L21 * U12;
+    const double b0 = rhs[0], b1 = rhs[1] - b0 * L10, b2 = rhs[2] - b0 * L20 - b1 * L21;
+    x[2] = b2 / U22;
+    x[1] = (b1 - U12 * x[2]) / U11;
+    x[0] = (b0 - A[1] * x[1] - A[2] * x[2]) / A[0];
+}
+
+int orc_lp2xyz(double line, double pixel, const orc_match_swath* s, const orc_match_orbit* o,
+               int burst, double* xyz)
+{
+    const double a = ORC_ELL_A, b = ORC_ELL_B;
+    const double az = s->burst_first_line_time[burst] + (line - 1.0) / s->prf;
+    const double rt = s->slr_first + (pixel - 1.0) / (s->rsr * 2);
+    double ps[3], vs[3];
+    orc_match_orbit_eval(o, az, ps, vs, NULL);
+    int it;
+    for (it = 0; it <= 10; it++) {
+        double d[3], eq[3], J[9], sol[3];
+        for (int k = 0; k < 3; k++) d[k] = xyz[k] - ps[k];
+        eq[0] = -orc_dot3(vs, d);
+        eq[1] = -(orc_dot3(d, d) - (ORC_SOL * rt) * (ORC_SOL * rt));
+        eq[2] = -((xyz[0] * xyz[0] + xyz[1] * xyz[1]) / (a * a) + (xyz[2] / b) * (xyz[2] / b) - 1.0);
+        J[0] = vs[0]; J[1] = vs[1]; J[2] = vs[2];
+        J[3] = 2 * d[0]; J[4] = 2 * d[1]; J[5] = 2 * d[2];
+        J[6] = 2 * xyz[0] / (a * a); J[7] = 2 * xyz[1] / (a * a); J[8] = 2 * xyz[2] / (b * b);
+        orc_solve33(sol, eq, J);
+        for (int k = 0; k < 3; k++) xyz[k] += sol[k];
+        if (fabs(sol[0]) < 1e-6 && fabs(sol[1]) < 1e-6 && fabs(sol[2]) < 1e-6) break;
+    }
+    return it;
+}
+
+/* zero-Doppler time on the other orbit of the centre pixel of burst `b` */
+static double orc_centre_zdt(const orc_match_swath* s, const orc_match_orbit* o,
+                             const orc_match_orbit* other, int b, double az0)
+{
+    double lat, lon, xyz[3];
+    orc_grid_ll(s, az0, (s->slr_first + s->slr_last) / 2.0, &lat, &lon);
+    orc_llh2xyz_wgs84(lat * ORC_PI / 180.0, lon * ORC_PI / 180.0, 0.0, xyz);
+    orc_lp2xyz(s->lpb / 2, s->spb / 2, s, o, b, xyz);
+    return orc_xyz2aztime(xyz, other, az0);
+}
+
+int orc_burst_overlap(const orc_match_swath* m, const orc_match_swath* s,
+                      const orc_match_orbit* mo, const orc_match_orbit* so, int in_b0,
+                      int in_bn, int* out)
+{
+    const int B0 = in_b0 - 1, BN = in_bn - 1;
+    if (B0 < 0) return -1;
+    const double az_m = m->burst_first_line_time[B0] + m->lpb / 2.0 * m->dt;
+    const double zdt = orc_centre_zdt(m, mo, so, B0, az_m);
+    int m0, mn, s0, sn;
+    if (zdt > s->last_line_time) return -2;
+    if (zdt < s->last_line_time && zdt > s->first_line_time) {
+        m0 = B0;
+        int cc = 0;
+        while (cc < s->n_bursts && zdt > s->burst_first_line_time[cc]) cc++;
+        s0 = cc - 1;
+        if (BN - m0 <= s->n_bursts - 1 - s0) {
+            mn = BN;
+            sn = s0 + (mn - m0);
+        } else {
+            sn = s->n_bursts - 1;
+            mn = m0 + (sn - s0);
+        }
+    } else {
+        const double az_s = s->burst_first_line_time[0] + s->lpb / 2 * s->dt;
+        const double zm = orc_centre_zdt(s, so, mo, 0, az_s);
+        if (zm < m->first_line_time) return -2;
+        if (!(zm < m->last_line_time && zm > m->first_line_time)) return -2;
+        s0 = 0;
+        int cc = 0;
+        while (cc < m->n_bursts && zm > m->burst_first_line_time[cc]) cc++;
+        m0 = cc - 1;
+        if (BN - m0 <= s->n_bursts - 1) {
+            mn = BN;
+            sn = s0 + (mn - m0);
+        } else {
+            sn = s->n_bursts - 1;
+            mn = m0 + (sn - s0);
+        }
+    }
+    out[0] = m0;
+    out[1] = mn;
+    out[2] = s0;
+    out[3] = sn;
+    return 0;
+}

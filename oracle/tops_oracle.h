@@ -142,6 +142,29 @@ int orc_deburst(const double* first_line_time, const double* last_line_time,
                 int burst0, int burstN, const void* gapped, int elem_bytes,
                 void* out, int* rows);
 
+/* --- burst matching: Func.cpp:1133-1260, 1349-1377, 1462-1584, 1610-1746 */
+typedef struct {
+    int n_bursts, lpb, spb;
+    double dt, prf, rsr, slr_first, slr_last, first_line_time, last_line_time;
+    const double* burst_first_line_time;
+    int geo_lines, geo_ppl;
+    const double *geo_az, *geo_rg, *geo_lat, *geo_lon;
+} orc_match_swath;
+typedef struct {
+    int num_points;
+    const double* t;
+    int ncoef;
+    const double *cx, *cy, *cz;
+} orc_match_orbit;
+double orc_xyz2aztime(const double* xyz, const orc_match_orbit* o, double t_init);
+int orc_lp2xyz(double line, double pixel, const orc_match_swath* s, const orc_match_orbit* o,
+               int burst, double* xyz);
+/* out = (m_burst0, m_burstN, s_burst0, s_burstN), 0-based; returns 0, or
+ * -1 (bad start burst) / -2 (no overlap) */
+int orc_burst_overlap(const orc_match_swath* m, const orc_match_swath* s,
+                      const orc_match_orbit* mo, const orc_match_orbit* so, int in_b0,
+                      int in_bn, int* out);
+
 #ifdef __cplusplus
 }
 #endif

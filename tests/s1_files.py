@@ -37,12 +37,14 @@ def seconds(u):
 
 def annotation_xml(sw, *, pol="VV", valid_first=None, valid_last=None, fvs_value=10,
                    lvs_value=None, geo_per_line=6, dc=None, dc_method="Data Analysis", fm=None,
-                   fm_form="polynomial", decorate=False):
+                   fm_form="polynomial", decorate=False, grid_ll=None):
     """Annotation XML text of swath `sw` (a synth.Swath).
 
     valid_first / valid_last: per burst, first and last line with valid
     samples (firstValidSample / lastValidSample = -1 outside); dc: list of
-    (az_time, t0, c0, c1, c2) estimates; fm: same for FM rates."""
+    (az_time, t0, c0, c1, c2) estimates; fm: same for FM rates; grid_ll:
+    (azimuth time, slant range m) -> (lat, lon) of the geolocation grid
+    points (a fixed made-up grid when None)."""
     nb, lpb, spb = sw.n_bursts, sw.lines_per_burst, sw.samples_per_burst
     if valid_first is None:
         valid_first = list(sw.first_valid_line)
@@ -143,11 +145,14 @@ def annotation_xml(sw, *, pol="VV", valid_first=None, valid_last=None, fvs_value
     a(f"    <geolocationGridPointList count='{npts}'>")
     for g in range(nb + 1):
         line = g * lpb
-        t = sw.first_line_time + line * dt
+        t = (sw.burst_first_line_time[g] if g < nb else sw.burst_last_line_time[nb - 1] + dt)
         for k in range(geo_per_line):
             px = round(k * (spb - 1) / (geo_per_line - 1))
-            lat = 30.0 + 0.1 * g - 0.01 * k
-            lon = 100.0 + 0.02 * k + 0.003 * g
+            if grid_ll is None:
+                lat = 30.0 + 0.1 * g - 0.01 * k
+                lon = 100.0 + 0.02 * k + 0.003 * g
+            else:
+                lat, lon = grid_ll(t, (sw.slr_time_first_pixel + px * sw.range_spacing / SOL) * SOL)
             inc = 30.0 + 0.5 * k
             a("      <geolocationGridPoint>")
             a(f"        <azimuthTime>{utc(t)}</azimuthTime>"
