@@ -147,23 +147,18 @@ tops_ctx* tops_ctx_create(const float* kernel_az, const float* kernel_rg, int np
         mom[4 * r + 3] = 0.0;
     }
     // azimuth rows placed at tap offsets 0..4 of a 16-row window (zeros
-    // elsewhere) and the range rows as double, for the fused walker
+    // elsewhere), for the fused walker
     std::vector<float> sh((size_t)kAzShifts * kLutRows * kAzShiftTaps, 0.0f);
     for (int o = 0; o < kAzShifts; o++)
         for (int r = 0; r < kLutRows; r++)
             for (int j = 0; j < kTaps; j++)
                 sh[((size_t)o * kLutRows + r) * kAzShiftTaps + o + j] = kernel_az[r * kTaps + j];
-    std::vector<double> rg64(kernel_rg, kernel_rg + (size_t)kLutRows * kTaps);
     hipError_t e = hipMalloc(&c->d_kaz, lut);
     if (e == hipSuccess) e = hipMalloc(&c->d_krg, lut);
     if (e == hipSuccess) e = hipMalloc(&c->d_mom, sizeof(double) * kLutRows * 4);
     if (e == hipSuccess) e = hipMalloc(&c->d_kaz_sh, sizeof(float) * sh.size());
-    if (e == hipSuccess) e = hipMalloc(&c->d_krg64, sizeof(double) * rg64.size());
     if (e == hipSuccess)
         e = hipMemcpy(c->d_kaz_sh, sh.data(), sizeof(float) * sh.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(c->d_krg64, rg64.data(), sizeof(double) * rg64.size(),
-                      hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_kaz, kernel_az, lut, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(c->d_krg, kernel_rg, lut, hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -183,7 +178,6 @@ void tops_ctx_destroy(tops_ctx* c)
     if (c->d_krg) (void)hipFree(c->d_krg);
     if (c->d_mom) (void)hipFree(c->d_mom);
     if (c->d_kaz_sh) (void)hipFree(c->d_kaz_sh);
-    if (c->d_krg64) (void)hipFree(c->d_krg64);
     c->scratch.release();
     delete c;
 }
@@ -225,7 +219,6 @@ int tops_resample_burst(tops_ctx* ctx, const tops_burst_geom* g, const tops_ci16
     p.krg = ctx->d_krg;
     p.mom = ctx->d_mom;
     p.kaz_sh = ctx->d_kaz_sh;
-    p.krg64 = ctx->d_krg64;
     p.az_shift_dev = nullptr;
     p.out = d_out;
     hipStream_t s = (hipStream_t)stream;

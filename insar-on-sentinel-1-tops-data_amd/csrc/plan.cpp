@@ -65,7 +65,6 @@ int fill_params(tops_plan* p)
         r.krg = p->ctx->d_krg;
         r.mom = p->ctx->d_mom;
         r.kaz_sh = p->ctx->d_kaz_sh;
-        r.krg64 = p->ctx->d_krg64;
         r.az_shift_dev = nullptr;
         r.out = it.out;
     }

@@ -520,24 +520,20 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
                 }
                 __syncthreads();
                 if (need && mybuf >= 0) {
-                    // this output's 12 range taps over U and PU
+                    // this output's 12 range taps over U and PU (the phase
+                    // sum takes the float taps widened to double, exactly)
                     float kp[kTaps];
                     load_taps(p.krg, fx, kp);
-                    const double2* kd = reinterpret_cast<const double2*>(p.krg64 + fx * kTaps);
                     const float2* u = &sm.u[mybuf][base];
                     const double* pu = &sm.pu[mybuf][base];
                     float sr = 0.f, si = 0.f;
                     double ph = 0.0;
 #pragma unroll
-                    for (int i = 0; i < kTaps; i += 2) {
-                        const double2 k2 = kd[i / 2];
-                        float2 v0 = u[i], v1 = u[i + 1];
-                        sr = fmaf(kp[i], v0.x, sr);
-                        si = fmaf(kp[i], v0.y, si);
-                        ph = fma(k2.x, pu[i], ph);
-                        sr = fmaf(kp[i + 1], v1.x, sr);
-                        si = fmaf(kp[i + 1], v1.y, si);
-                        ph = fma(k2.y, pu[i + 1], ph);
+                    for (int i = 0; i < kTaps; i++) {
+                        const float2 v = u[i];
+                        sr = fmaf(kp[i], v.x, sr);
+                        si = fmaf(kp[i], v.y, si);
+                        ph = fma((double)kp[i], pu[i], ph);
                     }
                     o = reramp(sr, si, ph);
                     need = false;

@@ -119,7 +119,6 @@ int tops_subswath_esd_coh(tops_ctx* ctx, const tops_subswath_desc* d, const tops
         p.krg = ctx->d_krg;
         p.mom = ctx->d_mom;
         p.kaz_sh = ctx->d_kaz_sh;
-        p.krg64 = ctx->d_krg64;
         p.out = d_reslave + (size_t)b * bpix;
         return p;
     };

@@ -47,7 +47,6 @@ struct tops_ctx {
     float* d_krg = nullptr;     // [2048 x 12] range LUT
     double* d_mom = nullptr;    // [2048 x 4] azimuth LUT moments M0 M1 M2 0
     float* d_kaz_sh = nullptr;  // [5 x 2048 x 16] azimuth rows shifted by 0..4 taps in 16
-    double* d_krg64 = nullptr;  // [2048 x 12] range LUT as double (exact)
     int npoints = 12;
     tops::Scratch scratch;      // FP64 baseline intermediates
 };
@@ -90,7 +89,6 @@ struct ResampleParams {
     const float* krg;
     const double* mom;
     const float* kaz_sh;    // tops_ctx::d_kaz_sh
-    const double* krg64;    // tops_ctx::d_krg64
     const double* az_shift_dev;   // device azimuth shift used instead of az_shift, or null
     tops_cf32* out;
 };
