@@ -384,8 +384,10 @@ int tops_deramp_and_resample(const tops_swath* m, const tops_swath* s, const dou
     if (!rc && !d_out) rc = d_o.ensure(mbytes);
     for (int b = mburst0; b <= mburstN && !rc; b++) {
         const int sbi = b + sburst0 - mburst0;
-        hipError_t e = hipMemcpy(d_s.p, s->slc + (size_t)sbi * lps * sps,
-                                 (size_t)lps * sps * sizeof(tops_ci16), hipMemcpyHostToDevice);
+        rc = staged_upload(d_s.p, s->slc + (size_t)sbi * lps * sps,
+                           (size_t)lps * sps * sizeof(tops_ci16));
+        if (rc) break;
+        hipError_t e = hipSuccess;
         double* par = d_par.as<double>();
         if (e == hipSuccess)
             e = hipMemcpy(par, s->doppler_rate + (size_t)sbi * sps, sizeof(double) * sps,
@@ -414,10 +416,7 @@ int tops_deramp_and_resample(const tops_swath* m, const tops_swath* s, const dou
         tops_cf32* dst = d_out ? d_out + (size_t)(b - mburst0) * lpm * spm : d_o.as<tops_cf32>();
         rc = tops_resample_burst(ctx, &g, d_s.as<tops_ci16>(), par, par + sps, par + 2 * sps, dst,
                                  TOPS_MODE_FAST, nullptr);
-        if (!rc && out) {
-            e = hipMemcpy(out + (size_t)(b - mburst0) * lpm * spm, dst, mbytes, hipMemcpyDeviceToHost);
-            if (e != hipSuccess) rc = hip_fail(e, "tops_deramp_and_resample readback", __FILE__, __LINE__);
-        }
+        if (!rc && out) rc = staged_download(out + (size_t)(b - mburst0) * lpm * spm, dst, mbytes);
     }
     if (!rc) {
         hipError_t e = hipDeviceSynchronize();
@@ -488,10 +487,11 @@ int tops_esd_and_coh(const tops_swath* m, const tops_swath* s, const double* cpm
         if ((rc = a.ensure(n * 4)) || (rc = b2.ensure(n * 4)) || (rc = c.ensure(n * 8)) ||
             (rc = d.ensure(n * 8)))
             return rc;
-        TOPS_HIP(hipMemcpy(a.p, mf.data(), n * 4, hipMemcpyHostToDevice));
-        TOPS_HIP(hipMemcpy(b2.p, mbk.data(), n * 4, hipMemcpyHostToDevice));
-        TOPS_HIP(hipMemcpy(c.p, sf.data(), n * 8, hipMemcpyHostToDevice));
-        TOPS_HIP(hipMemcpy(d.p, sbk.data(), n * 8, hipMemcpyHostToDevice));
+        if ((rc = staged_upload(a.p, mf.data(), n * 4)) ||
+            (rc = staged_upload(b2.p, mbk.data(), n * 4)) ||
+            (rc = staged_upload(c.p, sf.data(), n * 8)) ||
+            (rc = staged_upload(d.p, sbk.data(), n * 8)))
+            return rc;
         rc = tops_esd_shifts(a.as<tops_ci16>(), b2.as<tops_ci16>(), c.as<tops_cf32>(),
                              d.as<tops_cf32>(), sizes.data(), nov, spb, esd_win_az, esd_win_rg,
                              threshold, df, m->az_time_interval, sh.data(), nullptr, nullptr);
@@ -515,10 +515,10 @@ int tops_esd_and_coh(const tops_swath* m, const tops_swath* s, const double* cpm
         const int k = b - mburst0, sbi = b + sburst0 - mburst0;
         hipError_t e = hipSuccess;
         if (k == 0) {
-            e = hipMemcpy(d_res.p, reslave, bpix * sizeof(tops_cf32), hipMemcpyHostToDevice);
+            rc = staged_upload(d_res.p, reslave, bpix * sizeof(tops_cf32));
         } else {
-            e = hipMemcpy(d_s.p, s->slc + (size_t)sbi * lps * sps,
-                          (size_t)lps * sps * sizeof(tops_ci16), hipMemcpyHostToDevice);
+            rc = staged_upload(d_s.p, s->slc + (size_t)sbi * lps * sps,
+                               (size_t)lps * sps * sizeof(tops_ci16));
             double* par = d_par.as<double>();
             if (e == hipSuccess)
                 e = hipMemcpy(par, s->doppler_rate + (size_t)sbi * sps, sizeof(double) * sps,
@@ -529,7 +529,7 @@ int tops_esd_and_coh(const tops_swath* m, const tops_swath* s, const double* cpm
             if (e == hipSuccess)
                 e = hipMemcpy(par + 2 * sps, s->doppler_centroid + (size_t)sbi * sps,
                               sizeof(double) * sps, hipMemcpyHostToDevice);
-            if (e == hipSuccess) {
+            if (e == hipSuccess && !rc) {
                 tops_burst_geom g;
                 int mbox[4] = {0, spb - 1, b * lpb, (b + 1) * lpb - 1};
                 int sbox[4] = {0, sps - 1, sbi * lps, (sbi + 1) * lps - 1};
@@ -550,21 +550,18 @@ int tops_esd_and_coh(const tops_swath* m, const tops_swath* s, const double* cpm
         }
         if (e != hipSuccess) rc = hip_fail(e, "tops_esd_and_coh upload", __FILE__, __LINE__);
         if (rc) break;
-        if (reslave_esd) {
-            e = hipMemcpy(reslave_esd + (size_t)k * bpix, d_res.p, bpix * sizeof(tops_cf32),
-                          hipMemcpyDeviceToHost);
-            if (e != hipSuccess) { rc = hip_fail(e, "readback", __FILE__, __LINE__); break; }
-        }
+        if (reslave_esd &&
+            (rc = staged_download(reslave_esd + (size_t)k * bpix, d_res.p, bpix * sizeof(tops_cf32))))
+            break;
         if (coh) {
-            e = hipMemcpy(d_m.p, m->slc + (size_t)b * bpix, bpix * sizeof(tops_ci16),
-                          hipMemcpyHostToDevice);
-            if (e != hipSuccess) { rc = hip_fail(e, "upload", __FILE__, __LINE__); break; }
+            if ((rc = staged_upload(d_m.p, m->slc + (size_t)b * bpix, bpix * sizeof(tops_ci16))))
+                break;
             rc = tops_coherence(d_m.as<tops_ci16>(), d_res.as<tops_cf32>(), spb, lpb,
                                 (coh_win_rg - 1) / 2, (coh_win_az - 1) / 2, coh_win_rg, coh_win_az,
                                 d_coh.as<float>(), TOPS_MODE_FAST, nullptr);
             if (rc) break;
-            e = hipMemcpy(coh + (size_t)k * bpix, d_coh.p, bpix * sizeof(float), hipMemcpyDeviceToHost);
-            if (e != hipSuccess) { rc = hip_fail(e, "readback", __FILE__, __LINE__); break; }
+            if ((rc = staged_download(coh + (size_t)k * bpix, d_coh.p, bpix * sizeof(float))))
+                break;
         }
     }
     tops_ctx_destroy(ctx);

@@ -94,6 +94,11 @@ struct ResampleParams {
 };
 
 // launchers (resample.hip)
+// synchronous copies of large pageable host buffers through pinned staging
+// (ordered after the null stream's queued work)
+int staged_upload(void* d_dst, const void* h_src, size_t bytes);
+int staged_download(void* h_dst, const void* d_src, size_t bytes);
+
 int launch_resample_fp64(tops_ctx* ctx, const ResampleParams& p, hipStream_t s);
 int launch_resample_fast(const ResampleParams& p, hipStream_t s, int strip_rows = 0);
 int launch_resample_batch(const ResampleParams* d_items, const int4* d_work, int n_work,
