@@ -468,30 +468,25 @@ int tops_esd_and_coh(const tops_swath* m, const tops_swath* s, const double* cpm
                 return set_error(TOPS_ERR_ARG, "tops_esd_and_coh: overlap %d has %d lines", i,
                                  sizes[i]);
         double df = tops_spectral_sep(m->doppler_rate, m->n_bursts, spb, lpb, m->az_time_interval);
-        std::vector<tops_ci16> mf((size_t)total * spb), mbk((size_t)total * spb);
-        std::vector<tops_cf32> sf((size_t)total * spb), sbk((size_t)total * spb);
-        size_t off = 0;
-        for (int i = 0; i < nov; i++) {   // DataTypes.cpp:456-513 band gather
-            const int n = sizes[i];
-            const size_t fwd = (size_t)(i + 1) * lpb - n, bwd = (size_t)(i + 1) * lpb;
-            memcpy(&mf[off * spb], m->slc + ((size_t)mburst0 * lpb + fwd) * spb,
-                   sizeof(tops_ci16) * n * spb);
-            memcpy(&mbk[off * spb], m->slc + ((size_t)mburst0 * lpb + bwd) * spb,
-                   sizeof(tops_ci16) * n * spb);
-            memcpy(&sf[off * spb], reslave + fwd * spb, sizeof(tops_cf32) * n * spb);
-            memcpy(&sbk[off * spb], reslave + bwd * spb, sizeof(tops_cf32) * n * spb);
-            off += n;
-        }
+        // DataTypes.cpp:456-513 band gather: each band is contiguous in its
+        // raster, so it goes straight to its place in the device band array
         DBuf a, b2, c, d;
         size_t n = (size_t)total * spb;
         if ((rc = a.ensure(n * 4)) || (rc = b2.ensure(n * 4)) || (rc = c.ensure(n * 8)) ||
             (rc = d.ensure(n * 8)))
             return rc;
-        if ((rc = staged_upload(a.p, mf.data(), n * 4)) ||
-            (rc = staged_upload(b2.p, mbk.data(), n * 4)) ||
-            (rc = staged_upload(c.p, sf.data(), n * 8)) ||
-            (rc = staged_upload(d.p, sbk.data(), n * 8)))
-            return rc;
+        size_t off = 0;
+        for (int i = 0; i < nov && !rc; i++) {
+            const size_t cnt = (size_t)sizes[i] * spb;
+            const size_t fwd = (size_t)(i + 1) * lpb - sizes[i], bwd = (size_t)(i + 1) * lpb;
+            const tops_ci16* mr = m->slc + (size_t)mburst0 * lpb * spb;
+            if (!(rc = staged_upload(a.as<tops_ci16>() + off, mr + fwd * spb, cnt * 4)) &&
+                !(rc = staged_upload(b2.as<tops_ci16>() + off, mr + bwd * spb, cnt * 4)) &&
+                !(rc = staged_upload(c.as<tops_cf32>() + off, reslave + fwd * spb, cnt * 8)))
+                rc = staged_upload(d.as<tops_cf32>() + off, reslave + bwd * spb, cnt * 8);
+            off += cnt;
+        }
+        if (rc) return rc;
         rc = tops_esd_shifts(a.as<tops_ci16>(), b2.as<tops_ci16>(), c.as<tops_cf32>(),
                              d.as<tops_cf32>(), sizes.data(), nov, spb, esd_win_az, esd_win_rg,
                              threshold, df, m->az_time_interval, sh.data(), nullptr, nullptr);
