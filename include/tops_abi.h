@@ -473,6 +473,8 @@ typedef struct tops_annotation {
     double* burst_ll;                     /* [n_bursts][4] lat_min lat_max lon_min lon_max */
     double* dc;                           /* [n_bursts][4] t0 c0 c1 c2 (tops_deramp_params) */
     double* fm;                           /* [n_bursts][4] t0 c0 c1 c2 */
+    char first_line_utc[32];              /* productFirstLineUtcTime text (the   */
+    char last_line_utc[32];               /* orbit window's times, Main.cpp:39)  */
 } tops_annotation;
 int tops_annotation_read(const char* xml_path, tops_annotation* a);
 void tops_annotation_free(tops_annotation* a);

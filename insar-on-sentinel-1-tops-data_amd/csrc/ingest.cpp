@@ -363,12 +363,14 @@ int tops_annotation_read(const char* xml_path, tops_annotation* a)
                             "numberOfLines"));
     a->num_samples = atoi(txt({"imageAnnotation", "imageInformation", "numberOfSamples"},
                               "numberOfSamples"));
-    a->first_line_time = tops_utc2seconds(
-        txt({"imageAnnotation", "imageInformation", "productFirstLineUtcTime"},
-            "productFirstLineUtcTime"));
-    a->last_line_time = tops_utc2seconds(
-        txt({"imageAnnotation", "imageInformation", "productLastLineUtcTime"},
-            "productLastLineUtcTime"));
+    const char* flu = txt({"imageAnnotation", "imageInformation", "productFirstLineUtcTime"},
+                          "productFirstLineUtcTime");
+    const char* llu = txt({"imageAnnotation", "imageInformation", "productLastLineUtcTime"},
+                          "productLastLineUtcTime");
+    snprintf(a->first_line_utc, sizeof(a->first_line_utc), "%s", flu);
+    snprintf(a->last_line_utc, sizeof(a->last_line_utc), "%s", llu);
+    a->first_line_time = tops_utc2seconds(flu);
+    a->last_line_time = tops_utc2seconds(llu);
     a->az_time_interval = atof(txt({"imageAnnotation", "imageInformation", "azimuthTimeInterval"},
                                    "azimuthTimeInterval"));
     a->range_pixel_spacing = atof(txt({"imageAnnotation", "imageInformation", "rangePixelSpacing"},

@@ -109,7 +109,8 @@ class Annotation(C.Structure):
                 ("geo_lines", C.c_int), ("geo_points_per_line", C.c_int),
                 ("geo_az_time", _dp), ("geo_slant_range_time", _dp), ("geo_latitude", _dp),
                 ("geo_longitude", _dp), ("geo_incidence_angle", _dp),
-                ("near_range_on_left", C.c_int), ("burst_ll", _dp), ("dc", _dp), ("fm", _dp)]
+                ("near_range_on_left", C.c_int), ("burst_ll", _dp), ("dc", _dp), ("fm", _dp),
+                ("first_line_utc", C.c_char * 32), ("last_line_utc", C.c_char * 32)]
 
 
 class EofOrbit(C.Structure):

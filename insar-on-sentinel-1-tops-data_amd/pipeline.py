@@ -50,6 +50,93 @@ class SubswathRun:
         self.dem_geo = (lat0, lon0, dlat, dlon)
         self._keep = []
 
+    @classmethod
+    def from_files(cls, m_xml, s_xml, m_eof, s_eof, m_tiff, s_tiff, dem, dem_geo):
+        """A pair read from Sentinel-1 product files the way Main.cpp:26-82
+        reads it: each subswath's annotation XML (S1_Initialize and the DC /
+        FM lists of ComputeDerampPara), each orbit's EOF state vectors around
+        the acquisition (S1_OrbitInitialize) fitted with Polyfit_Orbit, the
+        CInt16 measurement TIFFs (burst-stacked), the LUT of InKernelInit.
+        The DEM is an in-memory grid (dem int32 [lines, pixels]; dem_geo =
+        lat0, lon0, dlat, dlon of post (0, 0))."""
+        from .synth import Swath
+        N = native
+        L = N.load()
+        self = cls.__new__(cls)
+        self.pair = None
+        self._keep = []
+        sws, slcs, self.orbits = {}, {}, {}
+        for key, xml, eof, tif in (("m", m_xml, m_eof, m_tiff), ("s", s_xml, s_eof, s_tiff)):
+            a = N.Annotation()
+            N.check(L.tops_annotation_read(str(xml).encode(), C.byref(a)), "tops_annotation_read")
+            try:
+                utcs = (a.first_line_utc.decode(), a.last_line_utc.decode())
+                nb, lpb = a.n_bursts, a.lines_per_burst
+                arr = lambda q, n: np.ctypeslib.as_array(q, shape=(n,)).copy()
+                sw = Swath(lines_per_burst=lpb, samples_per_burst=a.samples_per_burst,
+                           n_bursts=nb, az_time_interval=a.az_time_interval,
+                           range_spacing=a.range_pixel_spacing,
+                           slr_time_first_pixel=a.slr_time_first_pixel,
+                           wavelength=a.wavelength, prf=a.prf,
+                           azimuth_bandwidth=a.azimuth_bandwidth,
+                           range_sampling_rate=a.range_sampling_rate,
+                           steering_rate_deg=a.azimuth_steering_rate,
+                           first_valid_pixel=a.first_valid_pixel,
+                           burst_first_line_time=arr(a.burst_first_line_time, nb),
+                           burst_last_line_time=arr(a.burst_last_line_time, nb),
+                           dc=arr(a.dc, nb * 4).reshape(nb, 4), fm=arr(a.fm, nb * 4).reshape(nb, 4),
+                           burst_ll=arr(a.burst_ll, nb * 4).reshape(nb, 4),
+                           scene_line0=np.arange(nb, dtype=np.int64) * lpb,
+                           first_line_time=a.first_line_time, last_line_time=a.last_line_time,
+                           first_valid_line=arr(a.first_valid_line, nb),
+                           last_valid_line=arr(a.last_valid_line, nb), num_lines=a.num_lines)
+            finally:
+                L.tops_annotation_free(C.byref(a))
+            sws[key] = sw
+            first_utc, last_utc = utcs
+            # state vectors around the acquisition (the product's first and
+            # last line UTC times, Main.cpp:39-42)
+            o = N.EofOrbit()
+            N.check(L.tops_eof_read(str(eof).encode(), first_utc.encode(), last_utc.encode(),
+                                    C.byref(o)), "tops_eof_read")
+            try:
+                arr = lambda q: np.ctypeslib.as_array(q, shape=(o.n,)).copy()
+                t, x, y, z = arr(o.t), arr(o.x), arr(o.y), arr(o.z)
+            finally:
+                L.tops_eof_free(C.byref(o))
+            cx, cy, cz = N.polyfit_orbit(t, x, y, z)
+            speed = L.tops_orbit_speed(0.5 * (sw.first_line_time + sw.last_line_time),
+                                       t[len(t) // 2], N.ptr(cx), N.ptr(cy), N.ptr(cz), len(cx))
+            kt, tref, fdc, ka = N.deramp_params(sw, speed)
+            self.orbits[key] = dict(t=np.ascontiguousarray(t), cx=cx, cy=cy, cz=cz, kt=kt,
+                                    tref=tref, fdc=fdc, speed=speed)
+            # measurement raster: the burst-stacked CInt16 subswath
+            tf = L.tops_tiff_open(str(tif).encode())
+            if not tf:
+                N.check(L.tops_last_error(), "tops_tiff_open")
+            try:
+                w, h, dt_ = C.c_int(), C.c_int(), C.c_int()
+                N.check(L.tops_tiff_info(tf, C.byref(w), C.byref(h), C.byref(dt_)))
+                if dt_.value != N.GDT_CINT16 or w.value != sw.samples_per_burst or \
+                        h.value < sw.n_bursts * sw.lines_per_burst:
+                    raise ValueError(f"{tif}: {w.value} x {h.value} of type {dt_.value}")
+                rows = sw.n_bursts * sw.lines_per_burst
+                slc = np.empty((rows, w.value, 2), np.int16)
+                N.check(L.tops_tiff_read(tf, 0, 0, rows, w.value, N.GDT_CINT16,
+                                         slc.ctypes.data), "tops_tiff_read")
+            finally:
+                L.tops_tiff_close(tf)
+            slcs[key] = slc
+        self.m, self.s = sws["m"], sws["s"]
+        self.m_slc, self.s_slc = slcs["m"], slcs["s"]
+        self.nb = self.m.n_bursts
+        self.lpb, self.spb = self.m.lines_per_burst, self.m.samples_per_burst
+        self.kaz, self.krg = N.lut_init(12, self.m.prf, self.m.azimuth_bandwidth,
+                                        2 * self.m.range_sampling_rate)
+        self.dem = np.ascontiguousarray(dem, dtype=np.int32)
+        self.dem_geo = tuple(dem_geo)
+        return self
+
     # ---- ctypes structures ---------------------------------------------
     def swath(self, which):
         N = native

@@ -48,6 +48,8 @@ def test_annotation_fields(tops, sw, tmp_path):
     nb, lpb, spb, dt = sw.n_bursts, sw.lines_per_burst, sw.samples_per_burst, sw.az_time_interval
     try:
         assert a.polarisation == b"VV"
+        assert a.first_line_utc.decode() == S1.utc(sw.first_line_time)
+        assert a.last_line_utc.decode() == S1.utc(sw.last_line_time)
         assert (a.n_bursts, a.lines_per_burst, a.samples_per_burst) == (nb, lpb, spb)
         assert (a.num_lines, a.num_samples) == (nb * lpb, spb)
         assert a.az_time_interval == dt and a.prf == 1.0 / dt
