@@ -157,7 +157,8 @@ int esd_partial_capacity(const int* overlap_size, int n_overlap, int pixels)
 {
     int maxn = 0;
     for (int i = 0; i < n_overlap; i++) maxn = overlap_size[i] > maxn ? overlap_size[i] : maxn;
-    int tx = (pixels + TW_COLS - 1) / TW_COLS, ty = (maxn + TW_ROWS - 1) / TW_ROWS;
+    // the launch's grid: at least one tile row, also when every band is empty
+    int tx = (pixels + TW_COLS - 1) / TW_COLS, ty = maxn > 0 ? (maxn + TW_ROWS - 1) / TW_ROWS : 1;
     return n_overlap * tx * ty;
 }
 

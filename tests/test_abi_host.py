@@ -136,3 +136,19 @@ def test_reference_entry_points_clear_the_status(tops):
     assert L.tops_last_error() == N.TOPS_ERR_ARG
     L.EstESDShifts(None, None, None, None, None, None, 0, 0, 0, 6, 24, 0, 0.3, 1.0, 1.0)
     assert L.tops_last_error() == N.TOPS_OK
+
+
+def test_esd_workspace_covers_empty_bands(tops):
+    """The subswath workspace holds the ESD partials of the launch grid also
+    when every overlap band is empty (one tile row per band is still
+    launched and written)."""
+    N = tops.native
+    L = N.load()
+    for ov in ([0], [0, 0], [0, 100]):
+        o = np.ascontiguousarray(ov, np.int32)
+        d = N.SubswathDesc(len(ov) + 1, 400, 2048, 400, 2048, 2e-3, 2e-3, 10.0, N.ptr(o, C.c_int),
+                           6, 24, 0.3, 3, 10)
+        ws = L.tops_subswath_workspace_size(C.byref(d))
+        tiles_x = (2048 + 63) // 64
+        rows = max(1, (max(ov) + 15) // 16)
+        assert ws >= len(ov) * tiles_x * rows * 16 + len(ov) * 16, (ov, ws)

@@ -128,3 +128,72 @@ def test_subswath_device_deburst(orc, dev, run, flow):
                              0, run.nb - 1, gapped=src)
         print(f"  deburst {which}: {h} rows from line {y0}", flush=True)
         assert np.array_equal(got, ref)
+
+
+def _desc_run(tops, dev, run, ca, cr, nb, overlap, coh_win=(3, 10)):
+    """tops_subswath_esd_coh over the first nb bursts with the given overlap
+    line counts (the call's own description, not the swath's)."""
+    import ctypes as C
+    N = tops.native
+    L = N.load()
+    torch = dev.torch
+    lpb, spb = run.lpb, run.spb
+    m, s = run.m, run.s
+    so = run.orbits["s"]
+    put = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    d_m, d_s = put(run.m_slc[:nb * lpb]), put(run.s_slc[:nb * lpb])
+    d_kt, d_tref, d_fdc = (put(np.asarray(so[k][:nb], np.float64)) for k in ("kt", "tref", "fdc"))
+    ov = np.ascontiguousarray(overlap if len(overlap) else [0], np.int32)
+    df = L.tops_spectral_sep(N.ptr(run.orbits["m"]["kt"]), m.n_bursts, spb, lpb,
+                             m.az_time_interval)
+    desc = N.SubswathDesc(nb, lpb, spb, s.lines_per_burst, s.samples_per_burst,
+                          m.az_time_interval, s.az_time_interval, df, N.ptr(ov, C.c_int), 6, 24,
+                          0.3, coh_win[0], coh_win[1])
+    ws = L.tops_subswath_workspace_size(C.byref(desc))
+    assert ws > 0, N.last_error()
+    d_ws = torch.empty(ws, dtype=torch.uint8, device="cuda")
+    d_res = torch.empty(nb * lpb * spb * 8, dtype=torch.uint8, device="cuda")
+    d_coh = torch.empty(nb * lpb * spb * 4, dtype=torch.uint8, device="cuda")
+    d_sh = torch.full((max(nb - 1, 1),), 7.0, dtype=torch.float64, device="cuda")
+    ctx = L.tops_ctx_create(N.ptr(run.kaz, C.c_float), N.ptr(run.krg, C.c_float), 12)
+    try:
+        ca_, cr_ = np.ascontiguousarray(ca[:nb]), np.ascontiguousarray(cr[:nb])
+        N.check(L.tops_subswath_esd_coh(ctx, C.byref(desc), d_m.data_ptr(), d_s.data_ptr(),
+                                        d_kt.data_ptr(), d_tref.data_ptr(), d_fdc.data_ptr(),
+                                        N.ptr(ca_), N.ptr(cr_), d_res.data_ptr(),
+                                        d_coh.data_ptr(), d_sh.data_ptr() if nb > 1 else None,
+                                        d_ws.data_ptr(), ws, None), "tops_subswath_esd_coh")
+        torch.cuda.synchronize()
+    finally:
+        L.tops_ctx_destroy(ctx)
+    shape = (nb * lpb, spb)
+    return (d_res.cpu().numpy().view(np.complex64).reshape(shape),
+            d_coh.cpu().numpy().view(np.float32).reshape(shape), d_sh.cpu().numpy())
+
+
+def test_subswath_device_one_burst(tops, orc, dev, run, flow):
+    """One burst: no overlap, no ESD, no shift buffer; the result is pass 1
+    and its coherence."""
+    res, coh, _ = _desc_run(tops, dev, run, flow["ca"], flow["cr"], 1, [])
+    ref = oracle_resample_burst(orc, run, 0, flow["ca"][0], flow["cr"][0], 0.0,
+                                run.s.az_time_interval)
+    cref = orc.coherence(run.m_slc[:run.lpb], res.astype(np.complex128), 4, 1, 10, 3)
+    er, ec = rel_err(res, ref), float(np.max(np.abs(coh - cref)))
+    print(f"  one burst: resample {er:.2e}, coherence {ec:.2e}")
+    assert er <= 1e-5 and ec <= 1e-4
+
+
+def test_subswath_device_empty_overlap(tops, orc, dev, run, flow):
+    """Two bursts declared with a 0-line overlap: the band sums are empty,
+    the shift is 0 and burst 1 is resampled again with no shift but the
+    master's timing (Func.cpp:2701-2705)."""
+    res, coh, sh = _desc_run(tops, dev, run, flow["ca"], flow["cr"], 2, [0])
+    assert sh[0] == 0.0
+    lpb = run.lpb
+    ref = oracle_resample_burst(orc, run, 1, flow["ca"][1], flow["cr"][1], 0.0,
+                                run.m.az_time_interval)
+    er = rel_err(res[lpb:], ref)
+    cref = orc.coherence(run.m_slc[lpb:2 * lpb], res[lpb:].astype(np.complex128), 4, 1, 10, 3)
+    ec = float(np.max(np.abs(coh[lpb:] - cref)))
+    print(f"  empty overlap: burst 1 resample {er:.2e}, coherence {ec:.2e}")
+    assert er <= 1e-5 and ec <= 1e-4
