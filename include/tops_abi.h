@@ -222,8 +222,10 @@ int tops_plan_strip_rows(const tops_plan* plan);
 void tops_plan_destroy(tops_plan* plan);
 
 /* ESDAndCoh (Func.cpp:2496-2763) over one subswath held in HBM, with no
- * host round trip between its steps: pass-1 resample of every burst, ESD
- * over the overlap bands read in place, shifts formed on the device
+ * host round trip between its steps: pass-1 resample (burst 0 whole, of
+ * later bursts only the lines in an overlap band: pass 2 rewrites the
+ * rest, so the outputs equal a full pass 1's), ESD over the overlap bands
+ * read in place, shifts formed on the device
  * (|shift| > 100 -> 0), pass-2 resample of bursts 1..n-1 with
  * AzimuthShift = shift[b-1] (master lines per burst and azimuth interval,
  * Func.cpp:2701-2705), coherence of every burst.  All launches go to

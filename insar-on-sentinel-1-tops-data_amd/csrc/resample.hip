@@ -333,7 +333,8 @@ struct ColumnWalker {
 typedef __attribute__((address_space(4))) const float CFloat;
 typedef __attribute__((address_space(4))) const double CDouble;
 
-// One strip (sx, sy) of one burst.  p may live in global memory (batched
+// One strip of one burst: output columns sx*WK_OUT .. +WK_OUT-1, output rows
+// r0 .. r1-1 of the master box.  p may live in global memory (batched
 // launches): its fields are wave-uniform.
 struct WalkLds {
     float2 u[WK_KEYS][64];      // azimuth sums per key and column
@@ -343,13 +344,11 @@ struct WalkLds {
     double abc[3][64];          // per lane: the column's phase coefficients a, b, c
 };
 
-__device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, int sy,
-                                               int strip_rows, WalkLds& sm)
+__device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, int r0, int r1,
+                                               WalkLds& sm)
 {
     const int lane = threadIdx.x;
     const int c0 = sx * WK_OUT;
-    const int r0 = sy * strip_rows;
-    const int r1 = min(r0 + strip_rows, p.m_lines);
     const int c = c0 + lane;
     const bool out_lane = lane < WK_OUT && c < p.m_pixels;
 
@@ -555,10 +554,12 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     }
 }
 
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWalkWavesPerSimd))) resample_fast_kernel(ResampleParams p, int strip_rows)
+// Output rows row0 .. row_end-1 of the box, strip_rows per strip.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWalkWavesPerSimd))) resample_fast_kernel(ResampleParams p, int strip_rows, int row0, int row_end)
 {
     __shared__ WalkLds sm;
-    resample_strip(p, blockIdx.x, blockIdx.y, strip_rows, sm);
+    const int r0 = row0 + (int)blockIdx.y * strip_rows;
+    resample_strip(p, blockIdx.x, r0, min(r0 + strip_rows, row_end), sm);
 }
 
 // Batched form: one workgroup per entry of a strip table covering many
@@ -568,7 +569,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWalkWa
 {
     __shared__ WalkLds sm;
     const int4 w = work[blockIdx.x];
-    resample_strip(items[w.x], w.y, w.z, w.w, sm);
+    const int r0 = w.z * w.w;
+    resample_strip(items[w.x], w.y, r0, min(r0 + w.w, items[w.x].m_lines), sm);
 }
 
 int resample_strip_count(int m_lines, int m_pixels, int strip_rows, int* nx, int* ny)
@@ -618,13 +620,24 @@ int resample_rows_for_rounds(const int* lines, const int* pixels, int n, int rou
     return lo;
 }
 
-int launch_resample_fast(const ResampleParams& p, hipStream_t s, int strip_rows)
+int launch_resample_rows(const ResampleParams& p, int row0, int rows, hipStream_t s,
+                         int strip_rows)
 {
-    if (strip_rows <= 0) strip_rows = resample_rows_for_rounds(&p.m_lines, &p.m_pixels, 1, 1);
-    dim3 grid((p.m_pixels + WK_OUT - 1) / WK_OUT, (p.m_lines + strip_rows - 1) / strip_rows);
-    hipLaunchKernelGGL(resample_fast_kernel, grid, dim3(64), 0, s, p, strip_rows);
+    if (row0 < 0 || rows < 0 || row0 + rows > p.m_lines)
+        return set_error(TOPS_ERR_ARG, "resample: rows %d .. %d outside the box's %d lines", row0,
+                         row0 + rows - 1, p.m_lines);
+    if (rows == 0) return TOPS_OK;
+    if (strip_rows <= 0) strip_rows = resample_rows_for_rounds(&rows, &p.m_pixels, 1, 1);
+    dim3 grid((p.m_pixels + WK_OUT - 1) / WK_OUT, (rows + strip_rows - 1) / strip_rows);
+    hipLaunchKernelGGL(resample_fast_kernel, grid, dim3(64), 0, s, p, strip_rows, row0,
+                       row0 + rows);
     TOPS_HIP(hipGetLastError());
     return TOPS_OK;
+}
+
+int launch_resample_fast(const ResampleParams& p, hipStream_t s, int strip_rows)
+{
+    return launch_resample_rows(p, 0, p.m_lines, s, strip_rows);
 }
 
 int launch_resample_batch(const ResampleParams* d_items, const int4* d_work, int n_work,

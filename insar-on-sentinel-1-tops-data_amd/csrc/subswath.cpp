@@ -1,8 +1,10 @@
 // subswath.cpp -- ESDAndCoh (Func.cpp:2496-2763) over one subswath held in
 // HBM, with no host round trip between its steps:
-//   1. pass 1: every burst resampled onto its master burst (the
-//      DerampAndResample stage, Func.cpp:2326-2406: slave lines and slave
-//      azimuth interval);
+//   1. pass 1: the DerampAndResample stage (Func.cpp:2326-2406: slave lines
+//      and slave azimuth interval), on the rows this flow reads from it:
+//      burst 0 whole (its result is final) and, of every later burst, only
+//      the lines in an overlap band.  Pass 2 rewrites all other rows, so the
+//      outputs equal those of a full pass 1;
 //   2. ESD over the overlap bands, read in place from the master subswath
 //      and the pass-1 result (the forward band of overlap i is the last n_i
 //      lines of burst i, the backward band the first n_i lines of burst
@@ -123,7 +125,20 @@ int tops_subswath_esd_coh(tops_ctx* ctx, const tops_subswath_desc* d, const tops
         return p;
     };
 
-    for (int b = 0; b < nb && !rc; b++) rc = launch_resample_fast(params(b, false), s, 0);
+    // pass 1: burst 0 whole; burst b > 0 only its backward band (first
+    // n_{b-1} lines) and forward band (last n_b lines)
+    rc = launch_resample_fast(params(0, false), s, 0);
+    for (int b = 1; b < nb && !rc; b++) {
+        const int head = d->overlap_lines[b - 1];
+        const int tail = b + 1 < nb ? d->overlap_lines[b] : 0;
+        const ResampleParams p = params(b, false);
+        if (head + tail >= lpb) {
+            rc = launch_resample_fast(p, s, 0);
+        } else {
+            rc = launch_resample_rows(p, 0, head, s);
+            if (!rc) rc = launch_resample_rows(p, lpb - tail, tail, s);
+        }
+    }
     if (rc) return rc;
     if (nov > 0) {
         std::vector<int> f_off(nov), b_off(nov);

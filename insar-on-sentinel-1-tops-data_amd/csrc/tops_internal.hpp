@@ -101,6 +101,9 @@ int staged_download(void* h_dst, const void* d_src, size_t bytes);
 
 int launch_resample_fp64(tops_ctx* ctx, const ResampleParams& p, hipStream_t s);
 int launch_resample_fast(const ResampleParams& p, hipStream_t s, int strip_rows = 0);
+// output rows row0 .. row0+rows-1 of the master box only (the rest untouched)
+int launch_resample_rows(const ResampleParams& p, int row0, int rows, hipStream_t s,
+                         int strip_rows = 0);
 int launch_resample_batch(const ResampleParams* d_items, const int4* d_work, int n_work,
                           hipStream_t s);
 int resample_strip_count(int m_lines, int m_pixels, int strip_rows, int* nx, int* ny);
