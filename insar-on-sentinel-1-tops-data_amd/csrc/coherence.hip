@@ -57,7 +57,11 @@ __device__ __forceinline__ void coherence_tile(const tops_ci16* M, const tops_cf
 {
     const int r0 = ty * TW_ROWS, c0 = tx * TW_COLS;
     WindowSums ws[4];
-    tile_window_sums(M, S, w, h, r0, c0, dx, dy, win_rg, win_az, smem, ws, nullptr);
+    // (uniform branch; the launch's LDS, tile_window_lds_bytes, covers both)
+    if (tile_window_shared_ok(win_rg, win_az))
+        tile_window_sums_shared(M, S, w, h, r0, c0, dx, dy, win_rg, win_az, smem, ws, nullptr);
+    else
+        tile_window_sums(M, S, w, h, r0, c0, dx, dy, win_rg, win_az, smem, ws, nullptr);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int c = c0 + lane;
     if (c >= w) return;

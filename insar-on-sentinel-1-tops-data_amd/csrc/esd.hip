@@ -16,10 +16,11 @@
 // on the device by esd_shift_kernel for the device-resident two-pass flow.
 //
 // One fused kernel per call: each 16x64 tile computes both band coherences
-// from LDS (tile_window_sums), the double difference and the masked
-// phasors, sums them per thread with Kahan FP32, reduces the workgroup by
-// wave shuffles and writes one FP64 partial; a second kernel adds the
-// partials of each band in FP64 in a fixed order.
+// from LDS (tile_window_sums_shared for the reference's 6x24 window and
+// others it serves, tile_window_sums otherwise), the double difference and
+// the masked phasors, sums them per thread with Kahan FP32, reduces the
+// workgroup by wave shuffles and writes one FP64 partial; a second kernel
+// adds the partials of each band in FP64 in a fixed order.
 #include "tops_internal.hpp"
 #include "tile_window.hpp"
 
@@ -54,14 +55,27 @@ __device__ inline float wave_sum(float v)
     return v;
 }
 
+// Tile of workgroup i of n.  Workgroups go to the 8 XCDs in turn (i on XCD
+// i % 8); each XCD takes a contiguous range of tiles, band rows fastest, so
+// the neighbours whose halos overlap a tile's own run beside it and read
+// them from the same L2.
+__device__ inline int xcd_tile(int i, int n)
+{
+    const int x = i & 7, q = n >> 3, r = n & 7;
+    return x * q + min(x, r) + (i >> 3);
+}
+
+template <bool kShared>
 __global__ void __launch_bounds__(256) esd_band_kernel(EsdParams p)
 {
     extern __shared__ float4 smem[];
     __shared__ float red[4][2];
-    const int b = blockIdx.z;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int ty = t % p.tiles_y, tx = (t / p.tiles_y) % p.tiles_x;
+    const int b = t / (p.tiles_y * p.tiles_x);
     const int n = p.band[b].lines;
-    const int r0 = blockIdx.y * TW_ROWS, c0 = blockIdx.x * TW_COLS;
-    double2* out = p.partials + ((size_t)b * p.tiles_y + blockIdx.y) * p.tiles_x + blockIdx.x;
+    const int r0 = ty * TW_ROWS, c0 = tx * TW_COLS;
+    double2* out = p.partials + ((size_t)b * p.tiles_y + ty) * p.tiles_x + tx;
     if (r0 >= n) {   // whole workgroup leaves together: no barrier skipped
         if (threadIdx.x == 0) *out = make_double2(0.0, 0.0);
         return;
@@ -69,12 +83,34 @@ __global__ void __launch_bounds__(256) esd_band_kernel(EsdParams p)
     const size_t off_f = (size_t)p.band[b].f_offset * p.pixels;
     const size_t off_b = (size_t)p.band[b].b_offset * p.pixels;
     const int dh = (p.win_az - 1) / 2, dw = (p.win_rg - 1) / 2;
-    WindowSums wf[4], wb[4];
+    // the two bands in turn through one copy of the window code (a loop kept
+    // rolled, so both share its registers): coherence and centre product of
+    // each output sample
+    float coh_f[4], coh_b[4];
     float2 cf[4], cb[4];
-    tile_window_sums(p.mf + off_f, p.sf + off_f, p.pixels, n, r0, c0, dw, dh, p.win_rg,
-                     p.win_az, smem, wf, cf);
-    tile_window_sums(p.mb + off_b, p.sb + off_b, p.pixels, n, r0, c0, dw, dh, p.win_rg,
-                     p.win_az, smem, wb, cb);
+#pragma unroll 1
+    for (int band = 0; band < 2; band++) {
+        const tops_ci16* M = band ? p.mb + off_b : p.mf + off_f;
+        const tops_cf32* S = band ? p.sb + off_b : p.sf + off_f;
+        WindowSums ws[4];
+        float2 c[4];
+        if (kShared)
+            tile_window_sums_shared(M, S, p.pixels, n, r0, c0, dw, dh, p.win_rg, p.win_az, smem,
+                                    ws, c);
+        else
+            tile_window_sums(M, S, p.pixels, n, r0, c0, dw, dh, p.win_rg, p.win_az, smem, ws, c);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float co = ws[k].coherence();
+            if (band == 0) {
+                coh_f[k] = co;
+                cf[k] = c[k];
+            } else {
+                coh_b[k] = co;
+                cb[k] = c[k];
+            }
+        }
+    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float sx = 0.f, sy = 0.f, ex = 0.f, ey = 0.f;
     if (c0 + lane < p.pixels) {
@@ -82,19 +118,22 @@ __global__ void __launch_bounds__(256) esd_band_kernel(EsdParams p)
         for (int k = 0; k < 4; k++) {
             int r = r0 + wv * 4 + k;
             if (r >= n) break;
-            float coh = (wf[k].coherence() + wb[k].coherence()) / 2.0f;
+            float coh = (coh_f[k] + coh_b[k]) / 2.0f;
             if (coh < p.thr || coh >= 1.0f) continue;
-            // exp(j arg(a conj b)) from the unit phasors of a and b
-            float na = sqrtf(cf[k].x * cf[k].x + cf[k].y * cf[k].y);
-            float nb = sqrtf(cb[k].x * cb[k].x + cb[k].y * cb[k].y);
+            // exp(j arg(a conj b)) from the unit phasors of a and b, each
+            // scaled by the hardware reciprocal square root of its squared
+            // magnitude (about 1 ulp)
+            const float na2 = cf[k].x * cf[k].x + cf[k].y * cf[k].y;
+            const float nb2 = cb[k].x * cb[k].x + cb[k].y * cb[k].y;
             float px = 1.0f, py = 0.0f;   // atan2(0, 0) = 0
-            if (na > 0.0f && nb > 0.0f) {
-                float ax = cf[k].x / na, ay = cf[k].y / na;
-                float bx = cb[k].x / nb, by = cb[k].y / nb;
-                float dx = ax * bx + ay * by, dy = ay * bx - ax * by;
-                float nd = sqrtf(dx * dx + dy * dy);
-                px = dx / nd;
-                py = dy / nd;
+            if (na2 > 0.0f && nb2 > 0.0f) {
+                const float ia = __builtin_amdgcn_rsqf(na2), ib = __builtin_amdgcn_rsqf(nb2);
+                const float ax = cf[k].x * ia, ay = cf[k].y * ia;
+                const float bx = cb[k].x * ib, by = cb[k].y * ib;
+                const float dx = ax * bx + ay * by, dy = ay * bx - ax * by;
+                const float id = __builtin_amdgcn_rsqf(dx * dx + dy * dy);
+                px = dx * id;
+                py = dy * id;
             }
             kahan_add(sx, ex, px);
             kahan_add(sy, ey, py);
@@ -199,9 +238,13 @@ int launch_esd_bands(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* 
     p.tiles_x = (pixels + TW_COLS - 1) / TW_COLS;
     p.tiles_y = maxn > 0 ? (maxn + TW_ROWS - 1) / TW_ROWS : 1;
     p.partials = (double2*)d_partials;
-    dim3 grid(p.tiles_x, p.tiles_y, n_overlap);
-    hipLaunchKernelGGL(esd_band_kernel, grid, dim3(256), tile_window_lds_bytes(win_rg, win_az),
-                       st, p);
+    const dim3 grid(p.tiles_x * p.tiles_y * n_overlap);
+    if (tile_window_shared_ok(win_rg, win_az))
+        hipLaunchKernelGGL(esd_band_kernel<true>, grid, dim3(256),
+                           tile_window_shared_lds_bytes(win_rg, win_az), st, p);
+    else
+        hipLaunchKernelGGL(esd_band_kernel<false>, grid, dim3(256),
+                           tile_window_lds_bytes(win_rg, win_az), st, p);
     TOPS_HIP(hipGetLastError());
     hipLaunchKernelGGL(esd_final_kernel, dim3(n_overlap), dim3(256), 0, st,
                        (const double2*)d_partials, p.tiles_x * p.tiles_y, d_sums);

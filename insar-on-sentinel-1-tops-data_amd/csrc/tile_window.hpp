@@ -12,6 +12,9 @@
 //      reference's 24-sample window: no compensation needed for sums this
 //      short);
 //   3. each output adds winAz range sums.
+// tile_window_sums_shared computes the same window sums with the work of
+// neighbouring windows shared (runs of 8 range windows, 4 azimuth windows
+// per thread); every sum is still a plain sum of the window's own terms.
 #pragma once
 
 #include "tops_internal.hpp"
@@ -62,6 +65,76 @@ __device__ inline void kahan_add(float& s, float& comp, float v)
     s = t;
 }
 
+__device__ __forceinline__ float4 window_product(tops_ci16 m, tops_cf32 v)
+{
+    const float mr = m.re, mi = m.im;
+    return make_float4(mr * v.x + mi * v.y, mi * v.x - mr * v.y, v.x * v.x + v.y * v.y,
+                       mr * mr + mi * mi);
+}
+
+// Products P = (M conj S, |S|^2, |M|^2) of the PR x PC input window whose
+// first sample is (gr0, gc0), zero outside [0,h) x [0,w), to smem[PR*PC]
+// (row-major).  Each thread issues the loads of 8 positions before it uses
+// any of them; its position advances by 256 with no division in the loop.
+// A window inside the raster (most tiles) takes a loop without the bounds
+// tests.
+__device__ __forceinline__ void tile_products(const tops_ci16* M, const tops_cf32* S, int w, int h,
+                                     int gr0, int gc0, int PR, int PC, float4* smem)
+{
+    const int n = PR * PC;
+    const int step_r = 256 / PC, step_c = 256 - step_r * PC;
+    int rr = (int)threadIdx.x / PC, cc = (int)threadIdx.x - rr * PC;
+    if (gr0 >= 0 && gr0 + PR <= h && gc0 >= 0 && gc0 + PC <= w) {
+        const tops_ci16* Mt = M + (size_t)gr0 * w + gc0;
+        const tops_cf32* St = S + (size_t)gr0 * w + gc0;
+        for (int q0 = 0; q0 < n; q0 += 8 * 256) {
+            tops_ci16 m[8];
+            tops_cf32 v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (q0 + k * 256 + (int)threadIdx.x < n) {
+                    const int g = rr * w + cc;   // < PR * w: int
+                    m[k] = Mt[g];
+                    v[k] = St[g];
+                }
+                rr += step_r;
+                cc += step_c;
+                if (cc >= PC) { cc -= PC; rr++; }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int q = q0 + k * 256 + (int)threadIdx.x;
+                if (q < n) smem[q] = window_product(m[k], v[k]);
+            }
+        }
+        return;
+    }
+    for (int q0 = 0; q0 < n; q0 += 8 * 256) {
+        tops_ci16 m[8];
+        tops_cf32 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int q = q0 + k * 256 + (int)threadIdx.x;
+            const int gr = gr0 + rr, gc = gc0 + cc;
+            m[k].re = 0; m[k].im = 0;
+            v[k].x = 0.f; v[k].y = 0.f;
+            if (q < n && gr >= 0 && gr < h && gc >= 0 && gc < w) {
+                const size_t g = (size_t)gr * w + gc;
+                m[k] = M[g];
+                v[k] = S[g];
+            }
+            rr += step_r;
+            cc += step_c;
+            if (cc >= PC) { cc -= PC; rr++; }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int q = q0 + k * 256 + (int)threadIdx.x;
+            if (q < n) smem[q] = window_product(m[k], v[k]);
+        }
+    }
+}
+
 // Fills ws[k] for output rows r0 + 4*wave + k, column c0 + lane.  When
 // center != nullptr it also returns M conj S of each output sample itself.
 // Ends with a barrier, so smem may be reused right after.
@@ -73,23 +146,7 @@ __device__ inline void tile_window_sums(const tops_ci16* M, const tops_cf32* S, 
     const int PR = TW_ROWS + win_az - 1, PC = TW_COLS + win_rg - 1;
     float4* sP = smem;
     float4* sH = smem + PR * PC;
-    const int gr0 = r0 - dy, gc0 = c0 - dx;
-    for (int q = threadIdx.x; q < PR * PC; q += 256) {
-        int rr = q / PC, cc = q - rr * PC;
-        int gr = gr0 + rr, gc = gc0 + cc;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (gr >= 0 && gr < h && gc >= 0 && gc < w) {
-            size_t g = (size_t)gr * w + gc;
-            tops_ci16 m = M[g];
-            tops_cf32 s = S[g];
-            float mr = m.re, mi = m.im;
-            v.x = mr * s.x + mi * s.y;
-            v.y = mi * s.x - mr * s.y;
-            v.z = s.x * s.x + s.y * s.y;
-            v.w = mr * mr + mi * mi;
-        }
-        sP[q] = v;
-    }
+    tile_products(M, S, w, h, r0 - dy, c0 - dx, PR, PC, sP);
     __syncthreads();
     // range sums: direct sums of winRg products (no running subtraction, so
     // the error stays relative to the window's own sum of magnitudes)
@@ -118,6 +175,109 @@ __device__ inline void tile_window_sums(const tops_ci16* M, const tops_cf32* S, 
             float4 v = sP[(rr + dy) * PC + lane + dx];
             center[k] = make_float2(v.x, v.y);
         }
+    }
+    __syncthreads();
+}
+
+// row pitch (float4) of the range sums in tile_window_sums_shared: odd, so
+// the rows start in different LDS banks; PR x TW_HSTRIDE fits in the
+// PR x PC product tile (PC >= 70 for the windows it serves)
+constexpr int TW_HSTRIDE = TW_COLS + 1;
+
+__device__ __forceinline__ float4 add4(float4 a, float4 b)
+{
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+// Whether tile_window_sums_shared serves a window: its range runs need
+// winRg >= 7, its azimuth runs winAz >= 3, and its 8 range runs per product
+// row are one pass of the 256 threads (16 + winAz - 1 rows <= 32).
+__host__ __device__ inline bool tile_window_shared_ok(int win_rg, int win_az)
+{
+    return win_rg >= 7 && win_az >= 3 && win_az <= 17 &&
+           (size_t)(TW_ROWS + win_az - 1) * (TW_COLS + win_rg - 1) * sizeof(float4) <= TW_LDS_LIMIT;
+}
+
+// LDS bytes of tile_window_sums_shared (the range sums overwrite the
+// products, so only the product tile is held)
+__host__ __device__ inline size_t tile_window_shared_lds_bytes(int win_rg, int win_az)
+{
+    return (size_t)(TW_ROWS + win_az - 1) * (TW_COLS + win_rg - 1) * sizeof(float4);
+}
+
+// The window sums of tile_window_sums (same arguments and results; only
+// for windows tile_window_shared_ok accepts):
+//   range: task (row rr, run g) forms the 8 windows starting at product
+//     columns 8g .. 8g+7 from the suffix sums s_j = P[j] + .. + P[6], the
+//     shared middle P[7] + .. + P[winRg-1] and the prefix sums
+//     p_j = P[winRg] + .. + P[winRg+j-1]: window j = s_j + middle + p_j,
+//     31 product reads for 8 windows instead of 8 x 24;
+//   azimuth: the thread's 4 output rows share the range sums of rows
+//     3 .. winAz-1 in the same way (winAz + 3 reads for 4 outputs).
+// Ends with a barrier, so smem may be reused right after.
+__device__ inline void tile_window_sums_shared(const tops_ci16* M, const tops_cf32* S, int w,
+                                               int h, int r0, int c0, int dx, int dy,
+                                               int win_rg, int win_az, float4* smem,
+                                               WindowSums ws[4], float2* center)
+{
+    const int PR = TW_ROWS + win_az - 1, PC = TW_COLS + win_rg - 1;
+    tile_products(M, S, w, h, r0 - dy, c0 - dx, PR, PC, smem);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (center) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float4 v = smem[(wv * 4 + k + dy) * PC + lane + dx];
+            center[k] = make_float2(v.x, v.y);
+        }
+    }
+    // range sums of row rr = task % PR, windows 8g .. 8g+7 (g = task / PR):
+    // the lanes of a wave take different rows, whose starts (PC float4
+    // apart, PC odd) fall in different LDS banks, and the range sums are
+    // stored TW_HSTRIDE (odd) float4 apart per row for the same reason
+    const int task = threadIdx.x;
+    const bool has = task < PR * 8;
+    const int trow = task % PR, trun = task / PR;
+    float4 hs[8];
+    if (has) {
+        const float4* q = smem + trow * PC + trun * 8;
+        // the middle first: its reads do not overlap the suffix sums' registers
+        float4 mid = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = 7; i < win_rg; i++) mid = add4(mid, q[i]);
+        float4 sfx[7];
+        sfx[6] = q[6];
+#pragma unroll
+        for (int j = 5; j >= 0; j--) sfx[j] = add4(q[j], sfx[j + 1]);
+        const float4* t = q + win_rg;
+        hs[0] = add4(sfx[0], mid);
+        float4 pfx = t[0];
+#pragma unroll
+        for (int j = 1; j < 7; j++) {
+            hs[j] = add4(add4(sfx[j], mid), pfx);
+            pfx = add4(pfx, t[j]);
+        }
+        hs[7] = add4(mid, pfx);
+    }
+    __syncthreads();   // every product read: the range sums overwrite them
+    if (has) {
+        float4* hrow = smem + trow * TW_HSTRIDE + trun * 8;
+#pragma unroll
+        for (int j = 0; j < 8; j++) hrow[j] = hs[j];
+    }
+    __syncthreads();
+    // azimuth: output row 4wv+k sums range-sum rows 4wv+k .. 4wv+k+winAz-1
+    const float4* hc = smem + wv * 4 * TW_HSTRIDE + lane;
+    const float4 h0 = hc[0], h1 = hc[TW_HSTRIDE], h2 = hc[2 * TW_HSTRIDE];
+    const float4 s1 = add4(h1, h2), s0 = add4(h0, s1);
+    float4 mid = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int j = 3; j < win_az; j++) mid = add4(mid, hc[j * TW_HSTRIDE]);
+    const float4* ht = hc + win_az * TW_HSTRIDE;
+    const float4 p1 = ht[0], p2 = add4(p1, ht[TW_HSTRIDE]), p3 = add4(p2, ht[2 * TW_HSTRIDE]);
+    const float4 o[4] = {add4(s0, mid), add4(add4(s1, mid), p1), add4(add4(h2, mid), p2),
+                         add4(mid, p3)};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        ws[k].a0 = o[k].x; ws[k].a1 = o[k].y; ws[k].a2 = o[k].z; ws[k].a3 = o[k].w;
     }
     __syncthreads();
 }
