@@ -55,22 +55,14 @@ __device__ inline float wave_sum(float v)
     return v;
 }
 
-// Tile of workgroup i of n.  Workgroups go to the 8 XCDs in turn (i on XCD
-// i % 8); each XCD takes a contiguous range of tiles, band rows fastest, so
-// the neighbours whose halos overlap a tile's own run beside it and read
-// them from the same L2.
-__device__ inline int xcd_tile(int i, int n)
-{
-    const int x = i & 7, q = n >> 3, r = n & 7;
-    return x * q + min(x, r) + (i >> 3);
-}
-
 template <bool kShared>
 __global__ void __launch_bounds__(256) esd_band_kernel(EsdParams p)
 {
     extern __shared__ float4 smem[];
     __shared__ float red[4][2];
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    // tiles numbered band rows fastest and dealt to the XCDs in contiguous
+    // ranges: the neighbours whose halos overlap a tile's own run on its XCD
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int ty = t % p.tiles_y, tx = (t / p.tiles_y) % p.tiles_x;
     const int b = t / (p.tiles_y * p.tiles_x);
     const int n = p.band[b].lines;

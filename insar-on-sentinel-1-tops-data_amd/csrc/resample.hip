@@ -558,8 +558,13 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWalkWavesPerSimd))) resample_fast_kernel(ResampleParams p, int strip_rows, int row0, int row_end)
 {
     __shared__ WalkLds sm;
-    const int r0 = row0 + (int)blockIdx.y * strip_rows;
-    resample_strip(p, blockIdx.x, r0, min(r0 + strip_rows, row_end), sm);
+    // strips numbered row strips fastest and dealt to the XCDs in
+    // contiguous ranges: strips whose slave columns overlap run on one XCD
+    const int t = xcd_remap((int)(blockIdx.x + blockIdx.y * gridDim.x),
+                            (int)(gridDim.x * gridDim.y));
+    const int sx = t / (int)gridDim.y, sy = t - sx * (int)gridDim.y;
+    const int r0 = row0 + sy * strip_rows;
+    resample_strip(p, sx, r0, min(r0 + strip_rows, row_end), sm);
 }
 
 // Batched form: one workgroup per entry of a strip table covering many
@@ -568,7 +573,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWalkWa
                                                             const int4* work)
 {
     __shared__ WalkLds sm;
-    const int4 w = work[blockIdx.x];
+    const int4 w = work[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
     const int r0 = w.z * w.w;
     resample_strip(items[w.x], w.y, r0, min(r0 + w.w, items[w.x].m_lines), sm);
 }

@@ -58,6 +58,16 @@ namespace tops {
 // so it must be bit-identical wherever it is computed (positions on an
 // exact LUT-row boundary occur for offset fields with short decimal
 // coefficients).
+// Workgroup i of n runs on XCD i % 8 (the dispatcher deals workgroups to
+// the 8 XCDs in turn).  xcd_remap gives each XCD a contiguous range of
+// [0, n), so work items numbered next to each other (tiles or strips whose
+// halos overlap) run on one XCD and share its L2.  A bijection for any n.
+__device__ inline int xcd_remap(int i, int n)
+{
+    const int x = i & 7, q = n >> 3, r = n & 7;
+    return x * q + min(x, r) + (i >> 3);
+}
+
 __host__ __device__ inline double polyval6(double x, double y, const double* c)
 {
 #pragma clang fp contract(off)

@@ -197,3 +197,37 @@ def test_subswath_device_empty_overlap(tops, orc, dev, run, flow):
     ec = float(np.max(np.abs(coh[lpb:] - cref)))
     print(f"  empty overlap: burst 1 resample {er:.2e}, coherence {ec:.2e}")
     assert er <= 1e-5 and ec <= 1e-4
+
+
+def test_subswath_device_wide_overlaps(tops, orc, dev, run, flow):
+    """Overlaps declared wider than half a burst, so the two bands of the
+    middle burst cover all of its lines: pass 1 resamples that burst whole
+    (the other bursts only their band lines).  The shifts match the oracle's
+    ESD over its pass-1 bands and bursts 1..2 its pass-2 resample."""
+    nb, lpb, spb = 3, run.lpb, run.spb
+    m = run.m
+    ca, cr = flow["ca"], flow["cr"]
+    n = lpb // 2 + 5
+    res, coh, sh = _desc_run(tops, dev, run, ca, cr, nb, [n, n])
+    r1 = np.concatenate([oracle_resample_burst(orc, run, b, ca[b], cr[b], 0.0,
+                                               run.s.az_time_interval) for b in range(nb)])
+    sizes = np.array([n, n], np.int32)
+    M = run.m_slc
+    mf = np.concatenate([M[(i + 1) * lpb - n:(i + 1) * lpb] for i in range(nb - 1)])
+    mb = np.concatenate([M[(i + 1) * lpb:(i + 1) * lpb + n] for i in range(nb - 1)])
+    sf = np.concatenate([r1[(i + 1) * lpb - n:(i + 1) * lpb] for i in range(nb - 1)])
+    sb = np.concatenate([r1[(i + 1) * lpb:(i + 1) * lpb + n] for i in range(nb - 1)])
+    df = orc.spectral_sep(run.orbits["m"]["kt"], lpb, m.az_time_interval)
+    shifts, _ = orc.esd(mf, mb, sf, sb, sizes, spb, 6, 24, 0.3, df, m.az_time_interval)
+    shifts = np.where(np.abs(shifts) > 100, 0.0, shifts)
+    e = float(np.max(np.abs(sh - shifts)))
+    print(f"  wide overlaps ({n} lines): shifts gpu={sh} oracle={shifts} |d|={e:.2e}")
+    assert e <= 1e-4
+    assert rel_err(res[:lpb], r1[:lpb]) <= 1e-5
+    for b in (1, 2):
+        rows = slice(b * lpb, (b + 1) * lpb)
+        ref = oracle_resample_burst(orc, run, b, ca[b], cr[b], float(sh[b - 1]),
+                                    m.az_time_interval)
+        er = rel_err(res[rows], ref)
+        print(f"  wide overlaps: burst {b} resample {er:.2e}")
+        assert er <= 1e-5
