@@ -8,7 +8,8 @@ metric counts master-grid output samples per second.
 
 Printed JSON line (one): metric, value (Msamples/s), roofline of the
 dominant kernel (the fused resample kernel; HIP-event durations on the
-stream it runs on), the plain all-FP64 GPU form measured in the same run
+stream it runs on) in HBM bytes and, under "valu", in the reference form's
+FP32 arithmetic per sample, the plain all-FP64 GPU form measured in the same run
 (vs_plain_fp64), the error of the measured output against the FP64 oracle,
 and the CPU baseline (the FP64 oracle on the host cores).
 """
@@ -31,6 +32,14 @@ HBM_PEAK_GBS = 8000.0
 # CInt16 master read (4 B) and the float coherence write (4 B): 20 B.
 RESAMPLE_BYTES = 12
 PATH_BYTES = 20
+# the reference form's arithmetic per output sample: the 12x12 separable
+# complex-by-real sinc (12*12*2 + 12*2 FMA = 312 FMA, 624 flop) and the 12x12
+# interpolation of the deramp phase (156 FMA, 312 flop).  The kernel executes
+# the azimuth-first moment form, which needs fewer; this is the work it
+# replaces, priced against the FP32 vector peak (157.3 TFLOP/s counting packed
+# FP32; the build issues no packed FP32, which halves its own issue ceiling).
+REFERENCE_FLOP = 624 + 312
+FP32_VECTOR_PEAK_TFLOPS = 157.3
 
 
 def load_package():
@@ -657,7 +666,7 @@ def main():
         "vs_plain_fp64": round(fp64_step / step_s, 3),
         "plain_fp64_msamples_per_s": round(samples / fp64_step / 1e6, 2),
         "kernels_ms": {"resample_fast_kernel": round(rs_s * 1e3, 4),
-                       "coherence_fast_kernel": round(coh_s * 1e3, 4),
+                       "coherence_kernel": round(coh_s * 1e3, 4),
                        "plain_fp64_resample": round(fp64_rs * 1e3, 4),
                        "plain_fp64_coherence": round(fp64_coh * 1e3, 4)},
         "path_hbm_frac": round(PATH_BYTES * samples / step_s / 1e9 / HBM_PEAK_GBS, 4),
@@ -666,6 +675,10 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic("resample_fast_kernel", args.workload),
                      "algorithmic_bytes_per_launch": RESAMPLE_BYTES * samples},
+        "valu": {"kernel": "resample_fast_kernel", "reference_flop_per_sample": REFERENCE_FLOP,
+                 "achieved": round(REFERENCE_FLOP * samples / rs_s / 1e12, 2),
+                 "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(REFERENCE_FLOP * samples / rs_s / 1e12 / FP32_VECTOR_PEAK_TFLOPS, 4)},
     }
 
     if not args.no_cpu_baseline:
