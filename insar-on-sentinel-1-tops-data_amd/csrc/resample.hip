@@ -573,9 +573,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWalkWa
                                                             const int4* work)
 {
     __shared__ WalkLds sm;
-    const int4 w = work[xcd_remap((int)blockIdx.x, (int)gridDim.x)];
-    const int r0 = w.z * w.w;
-    resample_strip(items[w.x], w.y, r0, min(r0 + w.w, items[w.x].m_lines), sm);
+    // the table and the parameter blocks are read-only and read with
+    // wave-uniform indices: through the constant address space they become
+    // scalar loads, as the single-burst kernel's argument block is
+    typedef __attribute__((address_space(4))) const int CInt;
+    typedef __attribute__((address_space(4))) const ResampleParams CParams;
+    const CInt* w = (const CInt*)work + 4 * xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int item = w[0], sx = w[1], sy = w[2], rows = w[3];
+    CParams* it = (CParams*)items + item;
+    const int r0 = sy * rows;
+    resample_strip(*(const ResampleParams*)it, sx, r0, min(r0 + rows, it->m_lines), sm);
 }
 
 int resample_strip_count(int m_lines, int m_pixels, int strip_rows, int* nx, int* ny)
