@@ -281,7 +281,7 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
     }
     // per-burst launches (0), fixed batched heights, and the batched heights
     // that fill 1, 2 and 3 rounds of the GPU's walker wave slots
-    std::vector<int> cand = {0, 32, 64, 128, 256, 512};
+    std::vector<int> cand = {0, 32, 64, 128, 192, 256, 320, 384, 512};
     {
         std::vector<int> ls, ps;
         for (auto& it : p->items) {
