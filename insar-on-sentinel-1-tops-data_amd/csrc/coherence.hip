@@ -117,17 +117,29 @@ coherence_walk_kernel(const tops_ci16* M, const tops_cf32* S, int w, int h, floa
                       int rows)
 {
     __shared__ CohWalk3x10::Lds sm;
-    coherence_walk_block(M, S, w, h, coh, blockIdx.x, blockIdx.y, vec, rows, sm);
+    // blocks numbered bands fastest and dealt to the XCDs in contiguous
+    // ranges: bands whose halo columns overlap share an L2
+    const int t = xcd_remap((int)(blockIdx.x + blockIdx.y * gridDim.x),
+                            (int)(gridDim.x * gridDim.y));
+    const int by = t / (int)gridDim.x, bx = t - by * (int)gridDim.x;
+    coherence_walk_block(M, S, w, h, coh, bx, by, vec, rows, sm);
 }
 
 // Batched form: work[i] = {item, band, row block, vec}, CW_ROWS rows per block.
+// The table (bands fastest within a row block) is dealt to the XCDs in
+// contiguous ranges, so neighbouring blocks, whose halo rows and columns
+// overlap, share an L2; table and items are read through the constant
+// address space (scalar loads: read-only, wave-uniform indices).
 __global__ void __launch_bounds__(CW_THREADS) coherence_walk_batch_kernel(const CohItem* items,
                                                                           const int4* work)
 {
     __shared__ CohWalk3x10::Lds sm;
-    const int4 wk = work[blockIdx.x];
-    const CohItem& it = items[wk.x];
-    coherence_walk_block(it.m, it.s, it.w, it.h, it.coh, wk.y, wk.z, wk.w, CW_ROWS, sm);
+    typedef __attribute__((address_space(4))) const int CInt;
+    typedef __attribute__((address_space(4))) const CohItem CCohItem;
+    const CInt* wk = (const CInt*)work + 4 * xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int item = wk[0], band = wk[1], block = wk[2], vec = wk[3];
+    CCohItem* it = (CCohItem*)items + item;
+    coherence_walk_block(it->m, it->s, it->w, it->h, it->coh, band, block, vec, CW_ROWS, sm);
 }
 
 bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az)
