@@ -57,7 +57,7 @@ __device__ __forceinline__ void coherence_tile(const tops_ci16* M, const tops_cf
 {
     const int r0 = ty * TW_ROWS, c0 = tx * TW_COLS;
     WindowSums ws[4];
-    // (uniform branch; the launch's LDS, tile_window_lds_bytes, covers both)
+    // (uniform branch; the launch's LDS is tile_window_launch_lds for it)
     if (tile_window_shared_ok(win_rg, win_az))
         tile_window_sums_shared(M, S, w, h, r0, c0, dx, dy, win_rg, win_az, smem, ws, nullptr);
     else
@@ -197,12 +197,12 @@ int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_wor
         TOPS_HIP(hipGetLastError());
         return TOPS_OK;
     }
-    if (!tile_window_fits(win_rg, win_az))
+    const size_t lds = tile_window_launch_lds(win_rg, win_az);
+    if (lds == 0)
         return set_error(TOPS_ERR_ARG, "batched coherence: window %dx%d exceeds the LDS tile",
                          win_az, win_rg);
-    hipLaunchKernelGGL(coherence_batch_kernel, dim3(n_work), dim3(256),
-                       tile_window_lds_bytes(win_rg, win_az), st, d_items, d_work, dx, dy, win_rg,
-                       win_az);
+    hipLaunchKernelGGL(coherence_batch_kernel, dim3(n_work), dim3(256), lds, st, d_items, d_work,
+                       dx, dy, win_rg, win_az);
     TOPS_HIP(hipGetLastError());
     return TOPS_OK;
 }
@@ -210,7 +210,7 @@ int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_wor
 int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h, int dx,
                      int dy, int win_rg, int win_az, float* coh, int mode, hipStream_t st)
 {
-    if (mode == TOPS_MODE_FP64 || !tile_window_fits(win_rg, win_az)) {
+    if (mode == TOPS_MODE_FP64 || tile_window_launch_lds(win_rg, win_az) == 0) {
         dim3 g((w + 255) / 256, h);
         hipLaunchKernelGGL(coherence_fp64_kernel, g, dim3(256), 0, st, m, s, w, h, dx, dy,
                            win_rg, win_az, coh);
@@ -223,7 +223,7 @@ int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h, int d
     } else {
         dim3 g((w + TW_COLS - 1) / TW_COLS, (h + TW_ROWS - 1) / TW_ROWS);
         hipLaunchKernelGGL(coherence_fast_kernel, g, dim3(256),
-                           tile_window_lds_bytes(win_rg, win_az), st, m, s, w, h, dx, dy,
+                           tile_window_launch_lds(win_rg, win_az), st, m, s, w, h, dx, dy,
                            win_rg, win_az, coh);
     }
     TOPS_HIP(hipGetLastError());

@@ -10,7 +10,9 @@
 //                  shfl_vertical_ESD, CpCoh2_ESD)
 //   coh          : (coh_f + coh_b) / 2                       (Divide2)
 //   dd           : (Mf conj Sf) conj(Mb conj Sb)              (DoubleDifference4OneD)
-//   phasor       : 0 if coh < thr or coh >= 1, else exp(j arg dd) (MaskOut)
+//   phasor       : 0 if coh < thr or coh >= 1, else exp(j arg dd) (MaskOut;
+//                  a zero dd gives 1 or -1 by the sign of its real part,
+//                  as atan2 of signed zeros does there)
 //   sum_i        : sum of phasors over the band               (reduce0 x3)
 // The shift itself, atan2(sum_i) / (2 pi df dt), is formed on the host, or
 // on the device by esd_shift_kernel for the device-resident two-pass flow.
@@ -48,6 +50,21 @@ struct EsdParams {
     EsdBand band[kMaxEsdBands];
     double2* partials;   // [n_bands][tiles_y][tiles_x]
 };
+
+// a / |a| for a nonzero a: the squared magnitude scaled by the hardware
+// reciprocal square root (about 1 ulp); a tiny a is first scaled by 2^64
+// (exactly), so its squared magnitude stays a normal number
+__device__ inline float2 unit_phasor(float2 a)
+{
+    float n2 = a.x * a.x + a.y * a.y;
+    if (n2 < 1e-30f) {
+        a.x *= 0x1p64f;
+        a.y *= 0x1p64f;
+        n2 = a.x * a.x + a.y * a.y;
+    }
+    const float r = __builtin_amdgcn_rsqf(n2);
+    return make_float2(a.x * r, a.y * r);
+}
 
 __device__ inline float wave_sum(float v)
 {
@@ -112,20 +129,24 @@ __global__ void __launch_bounds__(256) esd_band_kernel(EsdParams p)
             if (r >= n) break;
             float coh = (coh_f[k] + coh_b[k]) / 2.0f;
             if (coh < p.thr || coh >= 1.0f) continue;
-            // exp(j arg(a conj b)) from the unit phasors of a and b, each
-            // scaled by the hardware reciprocal square root of its squared
-            // magnitude (about 1 ulp)
-            const float na2 = cf[k].x * cf[k].x + cf[k].y * cf[k].y;
-            const float nb2 = cb[k].x * cb[k].x + cb[k].y * cb[k].y;
-            float px = 1.0f, py = 0.0f;   // atan2(0, 0) = 0
-            if (na2 > 0.0f && nb2 > 0.0f) {
-                const float ia = __builtin_amdgcn_rsqf(na2), ib = __builtin_amdgcn_rsqf(nb2);
-                const float ax = cf[k].x * ia, ay = cf[k].y * ia;
-                const float bx = cb[k].x * ib, by = cb[k].y * ib;
-                const float dx = ax * bx + ay * by, dy = ay * bx - ax * by;
+            // exp(j arg(a conj b)), a = Mf Sf*, b = Mb Sb* (MaskOut's
+            // (cos, sin) of atan2 of the double difference)
+            float px, py;
+            if ((cf[k].x != 0.0f || cf[k].y != 0.0f) && (cb[k].x != 0.0f || cb[k].y != 0.0f)) {
+                // from the unit phasors of a and b
+                const float2 ua = unit_phasor(cf[k]), ub = unit_phasor(cb[k]);
+                const float dx = ua.x * ub.x + ua.y * ub.y, dy = ua.y * ub.x - ua.x * ub.y;
                 const float id = __builtin_amdgcn_rsqf(dx * dx + dy * dy);
                 px = dx * id;
                 py = dy * id;
+            } else {
+                // a or b is zero, so the double difference is a pair of
+                // signed zeros: atan2 gives +-0 (phasor 1) or +-pi (phasor
+                // -1) by the sign of its real part, which the products give
+                // as the reference's complex multiplications do
+                const float dr = cf[k].x * cb[k].x + cf[k].y * cb[k].y;
+                px = __builtin_signbit(dr) ? -1.0f : 1.0f;
+                py = 0.0f;
             }
             kahan_add(sx, ex, px);
             kahan_add(sy, ey, py);
@@ -213,7 +234,8 @@ int launch_esd_bands(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* 
     if (n_overlap <= 0) return TOPS_OK;
     if (n_overlap > kMaxEsdBands)
         return set_error(TOPS_ERR_ARG, "EstESDShifts: at most %d overlaps per call", kMaxEsdBands);
-    if (!tile_window_fits(win_rg, win_az))
+    const size_t lds = tile_window_launch_lds(win_rg, win_az);
+    if (lds == 0)
         return set_error(TOPS_ERR_ARG, "EstESDShifts: window %dx%d exceeds the LDS tile", win_az,
                          win_rg);
     EsdParams p;
@@ -232,11 +254,9 @@ int launch_esd_bands(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* 
     p.partials = (double2*)d_partials;
     const dim3 grid(p.tiles_x * p.tiles_y * n_overlap);
     if (tile_window_shared_ok(win_rg, win_az))
-        hipLaunchKernelGGL(esd_band_kernel<true>, grid, dim3(256),
-                           tile_window_shared_lds_bytes(win_rg, win_az), st, p);
+        hipLaunchKernelGGL(esd_band_kernel<true>, grid, dim3(256), lds, st, p);
     else
-        hipLaunchKernelGGL(esd_band_kernel<false>, grid, dim3(256),
-                           tile_window_lds_bytes(win_rg, win_az), st, p);
+        hipLaunchKernelGGL(esd_band_kernel<false>, grid, dim3(256), lds, st, p);
     TOPS_HIP(hipGetLastError());
     hipLaunchKernelGGL(esd_final_kernel, dim3(n_overlap), dim3(256), 0, st,
                        (const double2*)d_partials, p.tiles_x * p.tiles_y, d_sums);

@@ -205,6 +205,15 @@ __host__ __device__ inline size_t tile_window_shared_lds_bytes(int win_rg, int w
     return (size_t)(TW_ROWS + win_az - 1) * (TW_COLS + win_rg - 1) * sizeof(float4);
 }
 
+// LDS bytes a tile launch needs for the window code it will take
+// (tile_window_sums_shared where it serves the window, else
+// tile_window_sums), or 0 when neither fits the LDS limit
+__host__ __device__ inline size_t tile_window_launch_lds(int win_rg, int win_az)
+{
+    if (tile_window_shared_ok(win_rg, win_az)) return tile_window_shared_lds_bytes(win_rg, win_az);
+    return tile_window_fits(win_rg, win_az) ? tile_window_lds_bytes(win_rg, win_az) : 0;
+}
+
 // The window sums of tile_window_sums (same arguments and results; only
 // for windows tile_window_shared_ok accepts):
 //   range: task (row rr, run g) forms the 8 windows starting at product

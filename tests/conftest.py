@@ -118,3 +118,24 @@ def oracle_resample_burst(orc, run, b, ca, cr, shift, dt, row0=0, row1=None):
     box = [0, spb - 1, b * lpb, (b + 1) * lpb - 1]
     return orc.resample(run.s_slc[b * lpb:(b + 1) * lpb], box, box, b * lpb, dt, shift, ca, cr,
                         run.kaz, run.krg, kt, tref, fdc, row0, row1)
+
+
+def zero_master_band(n, P, seed):
+    """An overlap band whose forward master lines are invalid (all zero), as
+    at a real burst's edge: Mf = 0, Sf = (+-1, +-1) at random, and a
+    constant backward band with b = Mb Sb* = (-100, -100).  The forward
+    coherence is 0 (no master power), the backward 1, their mean 0.5, so
+    every sample passes the 0.3 mask with a = Mf Sf* a pair of signed zeros.
+    IEEE arithmetic (C99 F.9: the sign of a product of zeros is the xor of
+    the signs; a sum of opposite zeros is +0; atan2(+-0, -0) = +-pi,
+    atan2(+-0, +0) = +-0) makes the real part of dd = a b* equal to
+    -0, so the phasor -1, exactly where Re Sf > 0.  Returns the band and
+    the expected phasor sum."""
+    rng = np.random.default_rng(seed)
+    sf = rng.choice([-1.0, 1.0], size=(n, P)) + 1j * rng.choice([-1.0, 1.0], size=(n, P))
+    mf = np.zeros((n, P, 2), np.int16)
+    mb = np.zeros((n, P, 2), np.int16)
+    mb[..., 0] = 100
+    sb = np.full((n, P), -1.0 + 1.0j)   # b = 100 (-1 - 1j)
+    expect = float(np.sum(sf.real < 0) - np.sum(sf.real > 0))
+    return mf, mb, sf, sb, expect

@@ -85,6 +85,55 @@ def test_esd_device_and_reference_entry(tops, orc, dev, esd_case):
     assert np.max(np.abs(host - ref)) <= TOL_ESD
 
 
+@pytest.mark.parametrize("win", [(6, 24), (3, 7), (4, 9), (17, 31), (2, 5), (1, 3)])
+def test_esd_windows(tops, orc, dev, esd_case, win):
+    """Coherence windows other than the reference's 6 x 24: the shared window
+    sums (winAz >= 3 and winRg >= 7, up to 17 rows) and the direct sums
+    (smaller windows), against the oracle's shifts (1e-4 px) and band phasor
+    sums (|d| <= 1e-4 of the band's samples: a few threshold flips)."""
+    d = esd_case
+    N = tops.native
+    L = N.load()
+    waz, wrg = win
+    sizes = np.ascontiguousarray(d["sizes"], dtype=np.int32)
+    n = len(sizes)
+    ref, rsums = orc.esd(d["mf"], d["mb"], d["sf"].astype(np.complex128),
+                         d["sb"].astype(np.complex128), sizes, d["spb"], waz, wrg, 0.3, d["df"],
+                         d["dt"])
+    shifts, sums = np.zeros(n), np.zeros(2 * n)
+    dmf, dmb, dsf, dsb = (dev.put(a) for a in (d["mf"], d["mb"], d["sf"], d["sb"]))
+    N.check(L.tops_esd_shifts(dmf.data_ptr(), dmb.data_ptr(), dsf.data_ptr(), dsb.data_ptr(),
+                              sizes.ctypes.data_as(C.POINTER(C.c_int)), n, d["spb"], waz, wrg,
+                              0.3, d["df"], d["dt"], N.ptr(shifts), N.ptr(sums), None))
+    err = float(np.max(np.abs(shifts - ref)))
+    serr = np.abs(sums.reshape(n, 2) - np.asarray(rsums)).max(axis=1)
+    print(f"  esd window {waz}x{wrg}: shift |d| = {err:.2e} px, sum |d| = {serr}")
+    assert err <= TOL_ESD
+    # one sample masked or signed differently moves a sum by 1 or 2
+    assert np.all(serr <= 0.25)
+
+
+def test_esd_zero_products_take_the_sign_of_zero(tops, dev):
+    """Invalid (all-zero) forward master lines, as at a real burst's edge:
+    every sample passes the mask with a double difference of signed zeros,
+    whose phasor is -1 where Re Sf > 0 and +1 elsewhere (the reference's
+    atan2 of the products it forms); the band sum is that count exactly."""
+    from conftest import zero_master_band
+    N = tops.native
+    L = N.load()
+    n, P = 12, 2048
+    mf, mb, sf, sb, expect = zero_master_band(n, P, 5)
+    sizes = np.array([n], np.int32)
+    d = [dev.put(np.ascontiguousarray(a)) for a in (mf, mb, sf.astype(np.complex64),
+                                                     sb.astype(np.complex64))]
+    shifts, sums = np.zeros(1), np.zeros(2)
+    N.check(L.tops_esd_shifts(d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr(),
+                              sizes.ctypes.data_as(C.POINTER(C.c_int)), 1, P, 6, 24, 0.3, 5400.0,
+                              2.055556e-3, N.ptr(shifts), N.ptr(sums), None))
+    print(f"  zero forward master band: sums {sums}, expected ({expect}, 0)")
+    assert sums[0] == expect and sums[1] == 0.0
+
+
 def _orbit_coefs(tops, orbit):
     t, x, y, z = tops.synth.orbit_polyfit_inputs(orbit)
     return t, tops.native.polyfit_orbit(t, x, y, z)

@@ -152,7 +152,7 @@ def test_coherence_small_burst(tops, orc, dev, c1, rs):
     N = tops.native
     Lb = N.load()
     dm = dev.put(c1["M"])
-    for (waz, wrg) in [(3, 10), (6, 24), (1, 1), (5, 7)]:
+    for (waz, wrg) in [(3, 10), (6, 24), (1, 1), (5, 7), (17, 31)]:
         dx, dy = (wrg - 1) // 2, (waz - 1) // 2
         ref = orc.coherence(c1["M"], got.astype(np.complex128), dx, dy, wrg, waz)
         for mode in (0, 1):

@@ -19,6 +19,8 @@ ESD: a constructed double-difference phase th is recovered as
 th / (2 pi df dt); incoherent bands are masked to a zero sum.
 """
 import numpy as np
+
+from conftest import zero_master_band
 import pytest
 
 DT = 2.055556e-3
@@ -263,6 +265,15 @@ def test_esd_incoherent_masked(orc):
     q = lambda z: np.stack([np.round(z.real), np.round(z.imag)], -1).astype(np.int16)
     shifts, sums = orc.esd(q(rnd()), q(rnd()), rnd(), rnd(), [n], P, 6, 24, 0.9, 5400.0, DT)
     assert sums[0, 0] == 0 and sums[0, 1] == 0 and shifts[0] == 0
+
+
+def test_esd_zero_products_take_the_sign_of_zero(orc):
+    n, P = 12, 64
+    mf, mb, sf, sb, expect = zero_master_band(n, P, 5)
+    shifts, sums = orc.esd(mf, mb, sf, sb, [n], P, 6, 24, 0.3, 5400.0, DT)
+    assert sums[0, 0] == expect
+    assert abs(sums[0, 1]) < 1e-12   # sin(pi) = 1.2e-16 per -1 sample
+    assert 0 < np.sum(sf.real > 0) < n * P
 
 
 def test_esd_responds_to_injected_misregistration(tops, orc):
