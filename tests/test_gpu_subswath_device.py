@@ -36,6 +36,10 @@ def flow(run):
 
 
 def test_subswath_device_matches_oracle(orc, run, flow):
+    check_flow_against_oracle(orc, run, flow)
+
+
+def check_flow_against_oracle(orc, run, flow):
     m = run.m
     lpb, spb, nb = run.lpb, run.spb, run.nb
     ca, cr = flow["ca"], flow["cr"]
@@ -71,6 +75,44 @@ def test_subswath_device_matches_oracle(orc, run, flow):
         ec = float(np.max(np.abs(flow["coh"][rows] - cref)))
         print(f"  burst {b}: resample {er:.2e}, coherence {ec:.2e}")
         assert er <= 1e-5 and ec <= 1e-4
+
+
+def zero_invalid_lines(run):
+    """Zero each burst's lines outside [firstValidLine, lastValidLine] in
+    both rasters, as the measurement rasters of IW products hold them."""
+    for sw, slc in ((run.m, run.m_slc), (run.s, run.s_slc)):
+        lpb = sw.lines_per_burst
+        for b in range(sw.n_bursts):
+            slc[b * lpb:b * lpb + int(sw.first_valid_line[b])] = 0
+            slc[b * lpb + int(sw.last_valid_line[b]) + 1:(b + 1) * lpb] = 0
+
+
+@pytest.fixture(scope="module")
+def invalid_flow(tops):
+    import tops_amd.pipeline as pl
+    pair = tops.synth.subswath_iw1(n_bursts=3, lines=400, pixels=2048, delta=0.002)
+    run = pl.SubswathRun(pair)
+    zero_invalid_lines(run)
+    print("  zeroed invalid lines: geometric coreg", flush=True)
+    ca, cr = run.geometric_coreg()
+    ds = pl.DeviceSubswath(run, ca, cr, coh_win=(3, 10))
+    print("  zeroed invalid lines: device ESD and coherence", flush=True)
+    ds.enqueue()
+    res, coh, shifts = (a.copy() for a in ds.results())
+    yield run, dict(ds=ds, ca=ca, cr=cr, res=res, coh=coh, shifts=shifts)
+    ds.close()
+
+
+def test_subswath_device_invalid_lines(orc, invalid_flow):
+    """Invalid lines zeroed as in real products: the last lines of each
+    forward band and the first of each backward band have a zero master,
+    so thousands of samples pass the ESD mask (through the other band's
+    coherence) with a double difference of signed zeros.  Shifts, resampled
+    samples and coherence match the oracle as for filled bursts."""
+    run, flow = invalid_flow
+    m = run.m
+    assert int(m.first_valid_line[0]) > 0 and not run.m_slc[:int(m.first_valid_line[0])].any()
+    check_flow_against_oracle(orc, run, flow)
 
 
 def test_subswath_device_matches_host_stages(run, flow):
