@@ -59,6 +59,8 @@ def load():
         raise RuntimeError(f"{LIB_PATH} missing: run make -C oracle")
     L = C.CDLL(LIB_PATH)
     L.orc_lut_init.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _fp, _fp]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_max_threads.restype = C.c_int
     L.orc_polyfit_orbit.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
     L.orc_polyfit_orbit.restype = C.c_int
     for f in ("orc_orbit_pos", "orc_orbit_vel", "orc_orbit_acc"):
@@ -173,6 +175,15 @@ def deramp_params(nb, spb, slr0, rs, lpb, dt, fvp, dc, fm, steer, wl, speed):
     load().orc_deramp_params(nb, spb, slr0, rs, lpb, dt, fvp, dcs, fms, steer, wl, speed,
                              _p(fdc), _p(ka), _p(tref), _p(kt))
     return kt, tref, fdc, ka
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (timing only)."""
+    load().orc_set_threads(int(n))
+
+
+def max_threads():
+    return int(load().orc_max_threads())
 
 
 def resample(slave, mbox, sbox, first_line, dt, az_shift, cpm_az, cpm_rg, kaz, krg, kt, tref,

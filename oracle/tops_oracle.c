@@ -8,8 +8,16 @@
 #include "tops_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
+
+void orc_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
 
 #define ORC_PI 3.1415926535897931      /* Constants.h:8, CUDART_PI         */
 #define ORC_SOL 299792458.0            /* Constants.h:7, cuConstants.cuh:7 */

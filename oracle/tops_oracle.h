@@ -25,6 +25,11 @@ extern "C" {
 
 /* --- resample look-up table: Func.cpp:2105-2158, Constants.h:15-33 ------- */
 /* Fills kaz/krg [2048 x npoints] as the reference stores them (float).      */
+/* Threads of the oracle's parallel loops (used only to time the oracle at a
+ * chosen core count); the current maximum. */
+void orc_set_threads(int n);
+int orc_max_threads(void);
+
 void orc_lut_init(int npoints, double prf, double abw, double rsr2x,
                   float* kaz, float* krg);
 
