@@ -525,14 +525,28 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
                     load_taps(p.krg, fx, kp);
                     const float2* u = &sm.u[mybuf][base];
                     const double* pu = &sm.pu[mybuf][base];
+                    // in two groups of 6 taps, each group's LDS reads issued
+                    // together before its sums (one wait per group instead of
+                    // one per tap pair; all 12 at once would spill)
+                    static_assert(kTaps % 6 == 0, "tap groups of 6");
                     float sr = 0.f, si = 0.f;
                     double ph = 0.0;
 #pragma unroll
-                    for (int i = 0; i < kTaps; i++) {
-                        const float2 v = u[i];
-                        sr = fmaf(kp[i], v.x, sr);
-                        si = fmaf(kp[i], v.y, si);
-                        ph = fma((double)kp[i], pu[i], ph);
+                    for (int h = 0; h < kTaps; h += 6) {
+                        float2 uu[6];
+                        double pp[6];
+#pragma unroll
+                        for (int i = 0; i < 6; i++) {
+                            uu[i] = u[h + i];
+                            pp[i] = pu[h + i];
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int i = 0; i < 6; i++) {
+                            sr = fmaf(kp[h + i], uu[i].x, sr);
+                            si = fmaf(kp[h + i], uu[i].y, si);
+                            ph = fma((double)kp[h + i], pp[i], ph);
+                        }
                     }
                     o = reramp(sr, si, ph);
                     need = false;
