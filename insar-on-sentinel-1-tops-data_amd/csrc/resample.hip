@@ -487,6 +487,10 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
             static_assert(kLutRows == 2048, "key packing below");
             auto serve_round = [&](unsigned long long& h) {
                 int nk = 0, mybuf = -1;
+                // the column's phase coefficients, read once per round (their
+                // LDS reads then overlap the keys' scalar tap loads)
+                const double abc0 = sm.abc[0][lane], abc1 = sm.abc[1][lane];
+                const double abc2 = sm.abc[2][lane];
                 for (; h && nk < WK_KEYS; h &= h - 1) {
                     const int kq = __builtin_amdgcn_readlane(key, __builtin_ctzll(h));
                     const int iyq = (int)((unsigned)kq >> 11), fyq = kq & (kLutRows - 1);
@@ -512,8 +516,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
                     double bq = fma(y0, m.x, m.y);
                     double cq = y0 * y0 * m.x + 2.0 * y0 * m.y + m.z;
                     sm.u[nk][lane] = u;
-                    sm.pu[nk][lane] = fma(m.x, sm.abc[0][lane],
-                                          fma(bq, sm.abc[1][lane], cq * sm.abc[2][lane]));
+                    sm.pu[nk][lane] = fma(m.x, abc0, fma(bq, abc1, cq * abc2));
                     if (key == kq) mybuf = nk;
                     nk++;
                 }
