@@ -161,8 +161,8 @@ int coherence_walk_count(int w, int h, int* nx, int* ny)
     return *nx * *ny;
 }
 
-// Rows per workgroup for a single burst: 7 workgroups per CU in one round
-// (70 VGPRs allow 7 waves per SIMD, the LDS row buffers 9 workgroups per
+// Rows per workgroup for a single burst: 5 workgroups per CU in one round
+// (94 VGPRs allow 5 waves per SIMD, the LDS row buffers 9 workgroups per
 // CU), at least 16 rows.
 static int coherence_walk_rows(int w, int h)
 {
@@ -173,7 +173,7 @@ static int coherence_walk_rows(int w, int h)
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             cus <= 0)
             cus = 256;
-        slots = cus * 7;
+        slots = cus * 5;
     }
     const int nx = (w + CW_BAND - 1) / CW_BAND;
     const int ny = std::max(1, slots / nx);

@@ -141,12 +141,18 @@ struct CohWalk {
                         (amplitude_ok(cur.s.z * cur.s.z + cur.s.w * cur.s.w) ? 2u : 0u);
                 __syncthreads();
                 // window of output c: LDS entries 2t .. 2t+WR-1; of c+1: one more
+                // the WR + 1 products read together, before any is summed
+                // (one LDS wait per row instead of one per pair of reads)
                 const float4* rp = &sm.row[b][CW_PER_THREAD * t];
-                float4 mid = rp[1];
+                float4 v[WR + 1];
 #pragma unroll
-                for (int k = 2; k < WR; k++) mid = cw_add(mid, rp[k]);
-                H[q][0] = cw_add(rp[0], mid);
-                H[q][1] = cw_add(mid, rp[WR]);
+                for (int k = 0; k <= WR; k++) v[k] = rp[k];
+                __builtin_amdgcn_sched_barrier(0);
+                float4 mid = v[1];
+#pragma unroll
+                for (int k = 2; k < WR; k++) mid = cw_add(mid, v[k]);
+                H[q][0] = cw_add(v[0], mid);
+                H[q][1] = cw_add(mid, v[WR]);
                 if (gr - first < WA - 1) continue;
                 // output row r: input rows r-DY .. r-DY+WA-1 are in the buffer
                 const int r = gr - (WA - 1) + DY;
