@@ -11,7 +11,9 @@
 // Per input row each thread loads its own 2 samples (one 8-byte CInt16 pair,
 // one 16-byte CF32 pair; threads 0 .. WR-2 also load one halo sample of the
 // band), writes their products to an LDS row (double-buffered: one barrier
-// per row) and reads back the WR+1 products its two outputs need.  The two
+// per row; even and odd columns in separate arrays, so the reads are
+// conflict-free) and reads back the WR+1 products its two outputs need, all
+// before summing any.  The two
 // range sums share their WR-1 middle terms but each is still a plain sum of
 // WR products (no running subtraction, so the rounding error stays relative
 // to each window's own sum of magnitudes).  The WA latest rows of range sums
@@ -52,9 +54,18 @@ struct CohWalk {
     static constexpr int ROW = CW_BAND + WR - 1;    // products per LDS row
     static_assert(HALO_L + HALO_R <= CW_THREADS, "halo loads");
 
+    static constexpr int ROWH = (ROW + 1) / 2;    // products per column parity
+    // a row's products split by column parity (column k at ev[k/2] or
+    // od[k/2]): a thread's window reads, two columns apart per lane in one
+    // array, become consecutive 16-byte reads across the lanes of a wave
     struct Lds {
-        float4 row[2][ROW];
+        float4 ev[2][ROWH];
+        float4 od[2][ROWH];
     };
+    __device__ static float4& at(Lds& sm, int b, int k)
+    {
+        return (k & 1) ? sm.od[b][k >> 1] : sm.ev[b][k >> 1];
+    }
 
     // this thread's two samples of one row (zero outside the burst), plus
     // its halo sample when it has one
@@ -107,11 +118,11 @@ struct CohWalk {
     __device__ static void store(const Raw& raw, Lds& sm, int b)
     {
         const int t = threadIdx.x;
-        sm.row[b][HALO_L + CW_PER_THREAD * t] = cw_prod(raw.m0, raw.s.x, raw.s.y);
-        sm.row[b][HALO_L + CW_PER_THREAD * t + 1] = cw_prod(raw.m1, raw.s.z, raw.s.w);
+        at(sm, b, HALO_L + CW_PER_THREAD * t) = cw_prod(raw.m0, raw.s.x, raw.s.y);
+        at(sm, b, HALO_L + CW_PER_THREAD * t + 1) = cw_prod(raw.m1, raw.s.z, raw.s.w);
         if (t < HALO_L + HALO_R) {
             const int k = t < HALO_L ? t : HALO_L + CW_BAND + (t - HALO_L);
-            sm.row[b][k] = cw_prod(raw.mh, raw.sh.x, raw.sh.y);
+            at(sm, b, k) = cw_prod(raw.mh, raw.sh.x, raw.sh.y);
         }
     }
 
@@ -143,10 +154,13 @@ struct CohWalk {
                 // window of output c: LDS entries 2t .. 2t+WR-1; of c+1: one more
                 // the WR + 1 products read together, before any is summed
                 // (one LDS wait per row instead of one per pair of reads)
-                const float4* rp = &sm.row[b][CW_PER_THREAD * t];
+                // (columns 2t + k: the even ones at ev[t + k/2], the odd at
+                // od[t + k/2])
+                static_assert(CW_PER_THREAD == 2, "window start 2t is even");
                 float4 v[WR + 1];
 #pragma unroll
-                for (int k = 0; k <= WR; k++) v[k] = rp[k];
+                for (int k = 0; k <= WR; k++)
+                    v[k] = (k & 1) ? sm.od[b][t + (k >> 1)] : sm.ev[b][t + (k >> 1)];
                 __builtin_amdgcn_sched_barrier(0);
                 float4 mid = v[1];
 #pragma unroll
