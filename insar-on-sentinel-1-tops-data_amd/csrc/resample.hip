@@ -197,7 +197,7 @@ int launch_resample_fp64(tops_ctx* ctx, const ResampleParams& p, hipStream_t s)
 // Outputs whose key or window the shared pass cannot serve (more than
 // WK_KEYS keys in one row segment, or a window outside the strip's 64
 // columns) are computed by interior_direct.
-constexpr int WK_OUT = 50;     // output columns per strip (64-lane wave)
+constexpr int WK_OUT = 51;     // output columns per strip (64-lane wave): 51 + 11 taps + 2 slack
 constexpr int WK_WIN = 16;     // register window rows
 constexpr int WK_KEYS = 4;     // azimuth keys per row served by the shared pass
 constexpr int kWalkWavesPerSimd = 4;   // occupancy the walker is compiled for
@@ -371,7 +371,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     // slave column of this lane
     int iy0, fy0, ix0, fx0;
     slave_pos(p, r0, c0, iy0, fy0, ix0, fx0);
-    // one column of slack on the left (and 2 on the right) for the slave
+    // one column of slack on each side for the slave
     // column drifting along the strip; same value in every lane
     const int ucol0 = ix0 - kHalf - 1;
     ColumnWalker w;
