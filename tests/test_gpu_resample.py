@@ -124,6 +124,37 @@ def test_resample_azimuth_shear(orc, c1, rs, shear):
     assert e <= TOL_RESAMPLE
 
 
+@pytest.mark.parametrize("wh", [(2, 2), (7, 3), (51, 16), (52, 17), (103, 5)])
+def test_resample_and_coherence_small_boxes(tops, orc, dev, c1, rs, wh):
+    """Master boxes narrower than one strip, shorter than one tile, or one
+    column wider than a strip, inside the burst.  The burst's Cpm over a
+    box this small spreads the slave positions over the whole burst, so
+    keys change at almost every output and most windows leave the strip's
+    columns (the direct path); resample and coherence match the oracle."""
+    w, h = wh
+    L, P = c1["L"], c1["P"]
+    x0, y0 = 300, 200
+    mbox = [x0, x0 + w - 1, y0, y0 + h - 1]
+    sbox = [0, P - 1, 0, L - 1]
+    ref = _oracle(orc, c1, mbox, sbox, 0, 0.0, c1["ca"], c1["cr"])
+    got, dres = rs.run(c1["S"], mbox, sbox, 0, c1["dt"], 0.0, c1["ca"], c1["cr"], c1["kt"],
+                       c1["tref"], c1["fdc"], 0)
+    assert np.count_nonzero(ref) > 0
+    e = rel_err(got, ref)
+    N = tops.native
+    Lb = N.load()
+    Mb = np.ascontiguousarray(c1["M"][y0:y0 + h, x0:x0 + w])
+    dm = dev.put(Mb)
+    dc = dev.empty(w * h * 4)
+    N.check(Lb.tops_coherence(dm.data_ptr(), dres.data_ptr(), w, h, 4, 1, 10, 3, dc.data_ptr(),
+                              0, None), "tops_coherence")
+    coh = dev.get(dc, np.float32, (h, w))
+    cref = orc.coherence(Mb, got.astype(np.complex128), 4, 1, 10, 3)
+    ec = float(np.max(np.abs(coh - cref)))
+    print(f"  box {w}x{h}: resample max|dz|/rms = {e:.3e}, coherence max|dc| = {ec:.3e}")
+    assert e <= TOL_RESAMPLE and ec <= TOL_COH
+
+
 def test_resample_out_of_box_is_zero(orc, c1, rs):
     """Offsets pushing part of the burst outside the slave box: zero region
     and edge band must match the oracle exactly in structure."""
