@@ -242,8 +242,9 @@ __device__ inline void tile_window_sums_shared(const tops_ci16* M, const tops_cf
     }
     // range sums of row rr = task % PR, windows 8g .. 8g+7 (g = task / PR):
     // the lanes of a wave take different rows, whose starts (PC float4
-    // apart, PC odd) fall in different LDS banks, and the range sums are
-    // stored TW_HSTRIDE (odd) float4 apart per row for the same reason
+    // apart; PC is odd for even windows such as the reference's 24) fall in
+    // different LDS banks, and the range sums are stored TW_HSTRIDE (odd)
+    // float4 apart per row for the same reason
     const int task = threadIdx.x;
     const bool has = task < PR * 8;
     const int trow = task % PR, trun = task / PR;
