@@ -2,16 +2,20 @@
 """Throughput of the TOPS burst hot path on one MI355X.
 
 A step is one pass of deramp -> 12x12 sinc resample -> reramp -> coherence
-over one synthetic burst already resident in HBM (default workload: a full
-IW1 burst, 1500 az x 21000 rg CInt16, 3 az x 10 rg coherence window).  The
-metric counts master-grid output samples per second.
+over bursts already resident in HBM.  The default workload is the batched
+IW pair: 3 subswaths x 9 bursts (21000 / 25000 / 24500 range samples,
+1480-1520 lines each, every burst its own seeded scene, offsets and
+Doppler), 3 az x 10 rg coherence window, one tuned plan replayed as a
+hipGraph.  The metric counts master-grid output samples per second.
 
-Printed JSON line (one): metric, value (Msamples/s), roofline of the
-dominant kernel (the fused resample kernel; HIP-event durations on the
-stream it runs on) in HBM bytes and, under "valu", in the reference form's
-FP32 arithmetic per sample, the plain all-FP64 GPU form measured in the same run
-(vs_plain_fp64), the error of the measured output against the FP64 oracle,
-and the CPU baseline (the FP64 oracle on the host cores).
+Printed JSON line (one): metric, value (Msamples/s), the roofline of the
+dominant kernel (its algorithmic bytes over its HIP-event launch duration on
+the plan's stream; measured VALU issue utilisation beside it), the plain
+all-FP64 GPU form on the same bursts (vs_plain_fp64), the error of the
+measured output against the FP64 oracle, and the CPU baseline (the FP64
+oracle on the host cores, on a sample of the bursts).  Other workloads:
+--workload stack (48 slave bursts against one master), single_burst_iw1,
+single_burst_small, subswath.
 """
 import argparse
 import ctypes as C
@@ -187,117 +191,261 @@ def bench_esd(torch, M, S_res, L, N, pair, kt, steps, check):
     return out
 
 
-def run_iw_pair(args, tops, torch):
-    """27 uneven bursts (3 subswaths x 9; 21000 / 25000 / 24500 samples,
-    1480-1520 lines each) in one batched plan: one resample launch and one
-    coherence launch over all bursts, tuned strip height, replayed as one
-    hipGraph.  Compared with per-burst plain FP64 launches on the same GPU."""
+# algorithmic bytes per output sample of each launch kind of a plan:
+# resample (slave in, CF32 out), coherence (master, resampled slave, coherence)
+# and the fused launch (the whole path)
+KIND_BYTES = (RESAMPLE_BYTES, 16, PATH_BYTES)
+KIND_LABEL = ("resample", "coherence", "fused resample + coherence")
+SIMDS = 256 * 4          # MI355X: 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9         # peak engine clock
+
+
+def read_sq(kernel, workload):
+    """SQ counters per dispatch of a kernel (rocprofv3 --pmc, committed
+    profile summary), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_sq.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def run_batched(args, tops, torch):
+    """A batched workload through one tuned plan replayed as a hipGraph:
+      iw_pair: 27 bursts (3 subswaths x 9; 21000 / 25000 / 24500 samples,
+               1480-1520 lines), every burst its own scene, offsets and
+               Doppler (synth.iw_pair_bursts);
+      stack:   one 3-burst IW2 master (1500 x 25000) against 16 slaves, 48
+               slave bursts (synth.stack_bursts); the batch is the number of
+               slave bursts whose buffers fit in 90% of the free HBM.
+    A step is one replay: deramp -> resample -> reramp -> coherence of every
+    burst.  Compared with per-burst plain FP64 launches and per-burst fast
+    launches on the same GPU; the CPU baseline and the error are taken on a
+    sample of the bursts (one per burst width)."""
     N = tops.native
     L = N.load()
-    rng = np.random.default_rng(2026)
-    widths = (21000, 25000, 24500)
+    workload = args.workload
+    S_ = tops.synth
+    bursts = S_.iw_pair_bursts() if workload == "iw_pair" else S_.stack_bursts()
     dev = torch.device("cuda:0")
+    free, _ = torch.cuda.mem_get_info(dev)
+    per_burst = max(b.lines * b.width for b in bursts) * (4 + 8 + 4 + 4)
+    admitted = int(min(len(bursts), 0.9 * free // per_burst))
+    if admitted < len(bursts):
+        raise SystemExit(f"only {admitted} of {len(bursts)} bursts fit in HBM")
+    # bursts kept on the host for the CPU baseline and the error: the first
+    # burst of each width
+    sampled, seen = [], set()
+    for i, bb in enumerate(bursts):
+        if bb.width not in seen:
+            seen.add(bb.width)
+            sampled.append(i)
+    if len(sampled) == 1:
+        sampled.append(len(bursts) - 1)
+    m0 = bursts[0].pair.master
+    kaz, krg = N.lut_init(12, m0.prf, m0.azimuth_bandwidth, 2 * m0.range_sampling_rate)
     stream = torch.cuda.Stream(dev)
     sp = C.c_void_p(stream.cuda_stream)
-    keep, specs = [], []
-    kaz = krg = None
-    for sw, width in enumerate(widths):
-        lines = [int(v) for v in rng.integers(1480, 1521, size=9)]
-        pair = tops.synth.make_pair(f"iw{sw + 1}", max(lines), width, seed=300 + sw,
-                                    dem_spacing=1.0, dc_slope_total=10.0 + 5.0 * sw)
-        m = pair.master
-        if kaz is None:
-            kaz, krg = N.lut_init(12, m.prf, m.azimuth_bandwidth, 2 * m.range_sampling_rate)
-        kt, tref, fdc, _ = pair.params()
-        M = pair.burst(0, "master")
-        S = pair.burst(0, "slave")
-        d = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev)
-             for k, v in dict(S=S, M=M, kt=kt[0], tref=tref[0], fdc=fdc[0]).items()}
-        keep.append(d)
-        for b, nl in enumerate(lines):
-            ca, cr = tops.synth.cpm_from_field(pair.az_off, pair.rg_off, nl, width, 0)
-            specs.append((d, nl, width, ca, cr, m.az_time_interval))
-        print(f"[bench] subswath {sw + 1}: {width} samples, lines {lines}", file=sys.stderr,
-              flush=True)
-    ctx = L.tops_ctx_create(N.ptr(kaz, C.c_float), N.ptr(krg, C.c_float), 12)
-    items = (N.BurstItem * len(specs))()
-    outs = []
+    keep, host, masters = [], {}, {}
+    items = (N.BurstItem * len(bursts))()
     samples = 0
-    geoms = []
-    for i, (d, nl, width, ca, cr, dt) in enumerate(specs):
-        box = [0, width - 1, 0, nl - 1]
-        g = N.BurstGeom((C.c_int * 4)(*box), (C.c_int * 4)(*box), 0, width, dt, 0.0,
-                        (C.c_double * 6)(*ca), (C.c_double * 6)(*cr))
-        out = torch.empty(nl * width * 2, dtype=torch.float32, device=dev)
-        coh = torch.empty(nl * width, dtype=torch.float32, device=dev)
-        outs.append((out, coh))
-        geoms.append(g)
+    t0 = time.perf_counter()
+    for i, bb in enumerate(bursts):
+        S = bb.slave()
+        mkey = ("m", bb.index) if workload == "stack" else i
+        if mkey not in masters:
+            M = bb.master()
+            masters[mkey] = (M, torch.from_numpy(M).to(dev))
+        M, dM = masters[mkey]
+        kt, tref, fdc = bb.params()
+        ca, cr = bb.cpm()
+        d = {"S": torch.from_numpy(S).to(dev), "M": dM,
+             "kt": torch.from_numpy(kt).to(dev), "tref": torch.from_numpy(tref).to(dev),
+             "fdc": torch.from_numpy(fdc).to(dev),
+             "out": torch.empty(bb.lines * bb.width * 2, dtype=torch.float32, device=dev),
+             "coh": torch.empty(bb.lines * bb.width, dtype=torch.float32, device=dev)}
+        g = N.BurstGeom((C.c_int * 4)(*bb.box), (C.c_int * 4)(*bb.box), bb.y0, bb.width, bb.dt,
+                        0.0, (C.c_double * 6)(*ca), (C.c_double * 6)(*cr))
+        d["geom"] = g
         items[i] = N.BurstItem(g, d["S"].data_ptr(), d["kt"].data_ptr(), d["tref"].data_ptr(),
-                               d["fdc"].data_ptr(), out.data_ptr(), d["M"].data_ptr(),
-                               coh.data_ptr())
-        samples += nl * width
-    plan = L.tops_plan_create(ctx, items, len(specs), 3, 10, 0)
+                               d["fdc"].data_ptr(), d["out"].data_ptr(), dM.data_ptr(),
+                               d["coh"].data_ptr())
+        keep.append(d)
+        if i in sampled:
+            host[i] = dict(S=S, M=M, kt=kt, tref=tref, fdc=fdc, ca=ca, cr=cr)
+        samples += bb.lines * bb.width
+        if i % 9 == 8 or i == len(bursts) - 1:
+            print(f"[bench] {workload}: {i + 1}/{len(bursts)} bursts generated "
+                  f"({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+        if workload == "stack":
+            del S
+    ctx = L.tops_ctx_create(N.ptr(kaz, C.c_float), N.ptr(krg, C.c_float), 12)
+    plan = L.tops_plan_create(ctx, items, len(bursts), 3, 10, 0)
     if not plan:
         raise RuntimeError(N.last_error())
     best = C.c_int(0)
     N.check(L.tops_plan_tune(plan, sp, C.byref(best)), "tune")
+    names = [L.tops_plan_kernel_name(plan, k).decode() for k in range(3)]
     N.check(L.tops_plan_capture_graph(plan, sp), "graph")
+    print(f"[bench] {workload}: schedule {best.value} ({', '.join(n for n in names if n)})",
+          file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         N.check(L.tops_plan_run(plan, sp))
-    stream.synchronize()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         N.check(L.tops_plan_run(plan, sp))
-    stream.synchronize()
+    torch.cuda.synchronize()
     step_s = (time.perf_counter() - t0) / args.steps
+    # the kernels of the schedule, with HIP events on the plan's stream
+    ms = (C.c_float * 3)()
+    nl = (C.c_int * 3)()
+    N.check(L.tops_plan_time_launches(plan, sp, max(3, min(args.steps, 10)), ms, nl), "timing")
+    kernels_ms = {names[k]: round(ms[k], 4) for k in range(3) if nl[k] > 0}
+    dom = max((k for k in range(3) if nl[k] > 0), key=lambda k: ms[k])
+    dom_s = ms[dom] * 1e-3 / nl[dom]            # one launch of the dominant kind
+    dom_bytes = KIND_BYTES[dom] * samples / nl[dom]
+    achieved = dom_bytes / dom_s / 1e9
+    roof = {"kernel": names[dom], "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(names[dom], workload),
+            "algorithmic_bytes_per_launch": int(dom_bytes),
+            "algorithmic_bytes_per_sample": KIND_BYTES[dom], "launches_per_step": nl[dom],
+            "launch_ms": round(dom_s * 1e3, 4)}
+    sq = read_sq(names[dom], workload)
+    valu = {"kernel": names[dom]}
+    if sq and sq.get("SQ_INSTS_VALU"):
+        issue = sq["SQ_INSTS_VALU"] * 4.0 / (SIMDS * CLOCK_HZ * dom_s)
+        valu.update({"sq_insts_valu_per_launch": sq["SQ_INSTS_VALU"],
+                     "sq_insts_salu_per_launch": sq.get("SQ_INSTS_SALU"),
+                     "valu_lane_ops_per_sample": round(sq["SQ_INSTS_VALU"] * 64.0 * nl[dom]
+                                                       / samples, 1),
+                     "issue_frac": round(issue, 4),
+                     "note": "VALU issue utilisation: SQ_INSTS_VALU x 4 cycles per wave64 "
+                             "instruction / (1024 SIMDs x 2.4 GHz x the launch duration)"})
+        if issue > roof["frac"]:
+            roof["bound"] = "valu_issue"
+    roof["limiter"] = roof["bound"]
+    valu["reference_form_flop_per_sample"] = REFERENCE_FLOP
+    valu["reference_form_tflops"] = round(REFERENCE_FLOP * samples / nl[dom] / dom_s / 1e12, 2)
     # per-burst plain FP64 launches (same inputs)
-    nfp = max(1, args.fp64_steps)
-    for _ in range(1):
-        for (d, nl, width, *_r), g, (out, coh) in zip(specs, geoms, outs):
-            N.check(L.tops_resample_burst(ctx, C.byref(g), d["S"].data_ptr(), d["kt"].data_ptr(),
-                                          d["tref"].data_ptr(), d["fdc"].data_ptr(),
-                                          out.data_ptr(), N.MODE_FP64, sp))
-    stream.synchronize()
+    def fp64_pass():
+        for bb, d in zip(bursts, keep):
+            N.check(L.tops_resample_burst(ctx, C.byref(d["geom"]), d["S"].data_ptr(),
+                                          d["kt"].data_ptr(), d["tref"].data_ptr(),
+                                          d["fdc"].data_ptr(), d["out"].data_ptr(), N.MODE_FP64,
+                                          sp))
+            N.check(L.tops_coherence(d["M"].data_ptr(), d["out"].data_ptr(), bb.width, bb.lines,
+                                     4, 1, 10, 3, d["coh"].data_ptr(), N.MODE_FP64, sp))
+    fp64_pass()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(nfp):
-        for (d, nl, width, *_r), g, (out, coh) in zip(specs, geoms, outs):
-            N.check(L.tops_resample_burst(ctx, C.byref(g), d["S"].data_ptr(), d["kt"].data_ptr(),
-                                          d["tref"].data_ptr(), d["fdc"].data_ptr(),
-                                          out.data_ptr(), N.MODE_FP64, sp))
-            N.check(L.tops_coherence(d["M"].data_ptr(), out.data_ptr(), width, nl, 4, 1, 10, 3,
-                                     coh.data_ptr(), N.MODE_FP64, sp))
-    stream.synchronize()
-    fp64_s = (time.perf_counter() - t0) / nfp
-    # the same bursts as per-burst fast launches (no plan, no graph)
+    for _ in range(max(1, args.fp64_steps)):
+        fp64_pass()
+    torch.cuda.synchronize()
+    fp64_s = (time.perf_counter() - t0) / max(1, args.fp64_steps)
+    # per-burst fast launches (no plan, no graph)
     def per_burst():
-        for (d, nl, width, *_r), g, (out, coh) in zip(specs, geoms, outs):
-            N.check(L.tops_resample_burst(ctx, C.byref(g), d["S"].data_ptr(), d["kt"].data_ptr(),
-                                          d["tref"].data_ptr(), d["fdc"].data_ptr(),
-                                          out.data_ptr(), N.MODE_FAST, sp))
-            N.check(L.tops_coherence(d["M"].data_ptr(), out.data_ptr(), width, nl, 4, 1, 10, 3,
-                                     coh.data_ptr(), N.MODE_FAST, sp))
+        for bb, d in zip(bursts, keep):
+            N.check(L.tops_resample_burst(ctx, C.byref(d["geom"]), d["S"].data_ptr(),
+                                          d["kt"].data_ptr(), d["tref"].data_ptr(),
+                                          d["fdc"].data_ptr(), d["out"].data_ptr(), N.MODE_FAST,
+                                          sp))
+            N.check(L.tops_coherence(d["M"].data_ptr(), d["out"].data_ptr(), bb.width, bb.lines,
+                                     4, 1, 10, 3, d["coh"].data_ptr(), N.MODE_FAST, sp))
     per_burst()
-    stream.synchronize()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         per_burst()
-    stream.synchronize()
+    torch.cuda.synchronize()
     per_burst_s = (time.perf_counter() - t0) / args.steps
-    L.tops_plan_destroy(plan)
-    L.tops_ctx_destroy(ctx)
-    value = samples / step_s / 1e6
-    print(json.dumps({
-        "metric": METRIC, "value": round(value, 2), "unit": "Msamples/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "iw_pair", "bursts": len(specs), "samples": samples,
-                   "coh_window": [3, 10], "strip_rows": best.value, "hip_graph": True,
+    # the measured outputs: one more replay of the plan
+    N.check(L.tops_plan_run(plan, sp))
+    torch.cuda.synchronize()
+    result = {
+        "metric": METRIC, "value": round(samples / step_s / 1e6, 2), "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": workload, "bursts": len(bursts), "samples": samples,
+                   "widths": sorted({b.width for b in bursts}),
+                   "lines": [min(b.lines for b in bursts), max(b.lines for b in bursts)],
+                   "taps": 12, "coh_window": [3, 10], "schedule_strip_rows": best.value,
+                   "kernels": [n for n in names if n], "hip_graph": True,
+                   "admitted_batch": admitted, "hbm_free_gb": round(free / 1e9, 1),
+                   "precision": "FP32 sinc MACs and window sums, FP64 phase reduced mod 2pi",
                    "parallelism": "single GPU"},
         "vs_plain_fp64": round(fp64_s / step_s, 3),
         "plain_fp64_msamples_per_s": round(samples / fp64_s / 1e6, 2),
         "per_burst_launches_msamples_per_s": round(samples / per_burst_s / 1e6, 2),
-        "path_hbm_frac": round(PATH_BYTES * samples / step_s / 1e9 / HBM_PEAK_GBS, 4)}))
+        "kernels_ms": kernels_ms,
+        "path_hbm_frac": round(PATH_BYTES * samples / step_s / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": roof, "valu": valu,
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as orc
+        threads = orc.max_threads()
+        t_cpu, n_cpu, err, cerr = 0.0, 0, 0.0, 0.0
+        for i in sampled:
+            bb, h, d = bursts[i], host[i], keep[i]
+            def cpu_pass(r1):
+                ref = orc.resample(h["S"], bb.box, bb.box, bb.y0, bb.dt, 0.0, h["ca"], h["cr"],
+                                   kaz, krg, h["kt"], h["tref"], h["fdc"], 0, r1)
+                return ref, orc.coherence(h["M"][:r1], ref, 4, 1, 10, 3)
+            cpu_pass(32)   # warm-up
+            times = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                ref, cref = cpu_pass(bb.lines)
+                times.append(time.perf_counter() - t0)
+            t_cpu += float(np.median(times))
+            n_cpu += bb.lines * bb.width
+            got = d["out"].cpu().numpy().view(np.complex64).reshape(bb.lines, bb.width)
+            gcoh = d["coh"].cpu().numpy().reshape(bb.lines, bb.width)
+            err = max(err, float(np.max(np.abs(got - ref)) / np.sqrt(np.mean(np.abs(ref) ** 2))))
+            cerr = max(cerr, float(np.max(np.abs(gcoh - cref))))
+            print(f"[bench] oracle burst {i} ({bb.lines} x {bb.width}): {times[-1]:.2f} s",
+                  file=sys.stderr, flush=True)
+        bb, h = bursts[sampled[0]], host[sampled[0]]
+        r1 = 150
+        orc.set_threads(1)
+        t1 = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            orc.resample(h["S"], bb.box, bb.box, bb.y0, bb.dt, 0.0, h["ca"], h["cr"], kaz, krg,
+                         h["kt"], h["tref"], h["fdc"], 0, r1)
+            t1.append(time.perf_counter() - t0)
+        orc.set_threads(threads)
+        sample_txt = ", ".join(f"{bursts[i].lines}x{bursts[i].width}" for i in sampled)
+        result["cpu_baseline"] = {
+            "value": round(n_cpu / t_cpu / 1e6, 3), "unit": "Msamples/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(sampled)} of the {len(bursts)} bursts ({sample_txt}, one per burst "
+                      f"width), full deramp+resample+reramp+coherence in FP64 "
+                      f"(oracle/tops_oracle.c, OpenMP), median of 3 passes each; the rate "
+                      f"extrapolates to the whole workload by sample count "
+                      f"({samples / (n_cpu / t_cpu):.1f} s)",
+            "extrapolated": True, "cpu": cpu_model(),
+            "one_core": {"value": round(r1 * bb.width / float(np.median(t1)) / 1e6, 3),
+                         "cores": 1, "sample": f"rows 0..{r1 - 1} of burst {sampled[0]}, "
+                                               f"resample only, median of 3 passes"}}
+        result["error_vs_oracle"] = {
+            "bursts": sampled, "resample_max_abs_over_rms": err, "tol_resample": 1e-5,
+            "coherence_max_abs": cerr, "tol_coherence": 1e-4}
+    if workload == "iw_pair" and not args.no_side_stages:
+        print("[bench] geocoreg and ESD", file=sys.stderr, flush=True)
+        pair = bursts[0].pair
+        kt, tref, fdc, _ = pair.params()
+        got = keep[0]["out"].cpu().numpy().view(np.complex64).reshape(bursts[0].lines,
+                                                                      bursts[0].width)
+        result["geocoreg"] = bench_geocoreg(tops, torch, pair, L, N, 5, not args.no_cpu_baseline)
+        result["esd"] = bench_esd(torch, masters[0][0], got, L, N, pair, kt[0], 10,
+                                  not args.no_cpu_baseline)
+    L.tops_plan_destroy(plan)
+    L.tops_ctx_destroy(ctx)
+    print(json.dumps(result))
 
 
 def run_subswath(args, tops, torch):
@@ -416,133 +564,18 @@ def run_subswath(args, tops, torch):
                                 "hbm_frac": round(db_bytes / db_ms / 1e6 / 8000.0, 3)}}))
 
 
-def run_stack(args, tops, torch):
-    """One master (3 IW2 bursts of 1500 x 25000) against 16 slaves, each with
-    its own offset field (az 0-0.5 px, rg 0-3 px) and Doppler centroid
-    (0-60 Hz): 48 slave bursts resampled and given coherence against the
-    master in one tuned batched plan replayed as a hipGraph.  The batch is
-    the number of slave bursts whose buffers (slave in, CF32 out, coherence
-    out) fit in 90% of the free HBM; it is reported as admitted_batch."""
-    N = tops.native
-    L = N.load()
-    rng = np.random.default_rng(2027)
-    n_slaves, n_bursts, lines, width = 16, 3, 1500, 25000
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.Stream(dev)
-    sp = C.c_void_p(stream.cuda_stream)
-    per_item = lines * width * (4 + 8 + 4)
-    free, total = torch.cuda.mem_get_info(dev)
-    admitted = int(min(n_slaves * n_bursts, (0.9 * free - 3 * lines * width * 4) // per_item))
-    if admitted < n_slaves * n_bursts:
-        raise SystemExit(f"only {admitted} slave bursts fit in HBM; the stack needs "
-                         f"{n_slaves * n_bursts}")
-    t0 = time.perf_counter()
-    pairs = []
-    for k in range(n_slaves):
-        az = np.array([rng.uniform(0, 0.25), rng.uniform(0, 0.25) * 0.5, rng.uniform(0, 0.25) * 0.3,
-                       0.0, rng.uniform(0, 0.25) * 0.2, 0.0])
-        rg = np.array([rng.uniform(0, 1.5), rng.uniform(0, 1.5) * 0.5, rng.uniform(0, 1.5) * 0.3,
-                       0.0, rng.uniform(0, 1.5) * 0.2, 0.0])
-        pairs.append(tops.synth.make_pair("stack_iw2", lines, width, n_bursts=n_bursts,
-                                          overlap=100, seed=4100, stream=k, az_off=az, rg_off=rg,
-                                          dem_spacing=1.0, dc_slope_total=10.0,
-                                          slave_dc0=float(rng.uniform(0, 60))))
-    m = pairs[0].master
-    kaz, krg = N.lut_init(12, m.prf, m.azimuth_bandwidth, 2 * m.range_sampling_rate)
-    masters = [torch.from_numpy(pairs[0].burst(b, "master")).to(dev) for b in range(n_bursts)]
-    keep, specs = [], []
-    for k, pr in enumerate(pairs):
-        kt, tref, fdc, _ = pr.params()
-        for b in range(n_bursts):
-            d = {"S": torch.from_numpy(pr.burst(b, "slave")).to(dev)}
-            for name, a in (("kt", kt[b]), ("tref", tref[b]), ("fdc", fdc[b])):
-                d[name] = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-            ca, cr = pr.cpm(b)
-            box = [0, width - 1, b * lines, (b + 1) * lines - 1]
-            g = N.BurstGeom((C.c_int * 4)(*box), (C.c_int * 4)(*box), b * lines, width,
-                            m.az_time_interval, 0.0, (C.c_double * 6)(*ca),
-                            (C.c_double * 6)(*cr))
-            d["out"] = torch.empty(lines * width * 2, dtype=torch.float32, device=dev)
-            d["coh"] = torch.empty(lines * width, dtype=torch.float32, device=dev)
-            keep.append(d)
-            specs.append((k, b, g, d))
-        print(f"[bench] slave {k + 1}/{n_slaves} generated ({time.perf_counter() - t0:.0f} s)",
-              file=sys.stderr, flush=True)
-    ctx = L.tops_ctx_create(N.ptr(kaz, C.c_float), N.ptr(krg, C.c_float), 12)
-    items = (N.BurstItem * len(specs))()
-    for i, (k, b, g, d) in enumerate(specs):
-        items[i] = N.BurstItem(g, d["S"].data_ptr(), d["kt"].data_ptr(), d["tref"].data_ptr(),
-                               d["fdc"].data_ptr(), d["out"].data_ptr(), masters[b].data_ptr(),
-                               d["coh"].data_ptr())
-    samples = len(specs) * lines * width
-    plan = L.tops_plan_create(ctx, items, len(specs), 3, 10, 0)
-    if not plan:
-        raise RuntimeError(N.last_error())
-    best = C.c_int(0)
-    N.check(L.tops_plan_tune(plan, sp, C.byref(best)), "tune")
-    N.check(L.tops_plan_capture_graph(plan, sp), "graph")
-    for _ in range(args.warmup):
-        N.check(L.tops_plan_run(plan, sp))
-    stream.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        N.check(L.tops_plan_run(plan, sp))
-    stream.synchronize()
-    step_s = (time.perf_counter() - t0) / args.steps
-    # plain FP64 per-burst launches over the same 48 bursts
-    t0 = time.perf_counter()
-    for k, b, g, d in specs:
-        N.check(L.tops_resample_burst(ctx, C.byref(g), d["S"].data_ptr(), d["kt"].data_ptr(),
-                                      d["tref"].data_ptr(), d["fdc"].data_ptr(),
-                                      d["out"].data_ptr(), N.MODE_FP64, sp))
-        N.check(L.tops_coherence(masters[b].data_ptr(), d["out"].data_ptr(), width, lines, 4, 1,
-                                 10, 3, d["coh"].data_ptr(), N.MODE_FP64, sp))
-    stream.synchronize()
-    fp64_s = time.perf_counter() - t0
-    # error of the fast path on slabs of the last slave's bursts
-    N.check(L.tops_plan_run(plan, sp))
-    stream.synchronize()
-    err = None
-    if not args.no_cpu_baseline:
-        from oracle import oracle as orc
-        k, b, g, d = specs[-1]
-        pr = pairs[k]
-        kt, tref, fdc, _ = pr.params()
-        ca, cr = pr.cpm(b)
-        box = [0, width - 1, b * lines, (b + 1) * lines - 1]
-        S = pr.burst(b, "slave")
-        got = d["out"].cpu().numpy().view(np.complex64).reshape(lines, width)
-        err = 0.0
-        for r0 in (0, 744, 1488):
-            ref = orc.resample(S, box, box, b * lines, m.az_time_interval, 0.0, ca, cr, kaz, krg,
-                               kt[b], tref[b], fdc[b], r0, r0 + 12)
-            rms = np.sqrt(np.mean(np.abs(ref) ** 2))
-            err = max(err, float(np.max(np.abs(got[r0:r0 + 12] - ref)) / rms))
-    L.tops_plan_destroy(plan)
-    L.tops_ctx_destroy(ctx)
-    print(json.dumps({
-        "metric": METRIC, "value": round(samples / step_s / 1e6, 2), "unit": "Msamples/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(step_s * 1e3, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "stack", "slaves": n_slaves, "bursts_per_slave": n_bursts,
-                   "burst": [lines, width], "samples": samples, "coh_window": [3, 10],
-                   "strip_rows": best.value, "hip_graph": True, "admitted_batch": admitted,
-                   "hbm_free_gb": round(free / 1e9, 1), "parallelism": "single GPU"},
-        "vs_plain_fp64": round(fp64_s / step_s, 3),
-        "plain_fp64_msamples_per_s": round(samples / fp64_s / 1e6, 2),
-        "path_hbm_frac": round(PATH_BYTES * samples / step_s / 1e9 / HBM_PEAK_GBS, 4),
-        "error_vs_oracle": {"resample_max_abs_over_rms_slabs": err, "tol_resample": 1e-5}}))
-
-
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="single_burst_iw1")
+    ap.add_argument("--workload", default="iw_pair",
+                    help="iw_pair (default: 27 uneven bursts, one tuned plan), stack, "
+                         "single_burst_iw1, single_burst_small, subswath")
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-side-stages", action="store_true",
+                    help="iw_pair: skip the geocoreg and ESD figures")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (twice) and exit: "
                          "the reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -565,10 +598,8 @@ def main():
         return
 
     tops = load_package()
-    if args.workload == "iw_pair":
-        return run_iw_pair(args, tops, torch)
-    if args.workload == "stack":
-        return run_stack(args, tops, torch)
+    if args.workload in ("iw_pair", "stack"):
+        return run_batched(args, tops, torch)
     if args.workload == "subswath":
         return run_subswath(args, tops, torch)
     N = tops.native

@@ -219,6 +219,16 @@ int tops_plan_capture_graph(tops_plan* plan, void* stream);
  * TOPS_TUNE_CACHE when set. */
 int tops_plan_tune(tops_plan* plan, void* stream, int* best_strip_rows);
 int tops_plan_strip_rows(const tops_plan* plan);
+/* Durations of the plan's launches: `reps` eager runs of its schedule on
+ * `stream` with an event pair around every launch.  ms[k] (k = 0 resample,
+ * 1 coherence, 2 fused resample + coherence; 3 entries) gets the mean per
+ * run of the summed durations of that kind's launches, launches[k]
+ * (optional, 3 entries) their count per run. */
+int tops_plan_time_launches(tops_plan* plan, void* stream, int reps, float* ms,
+                            int* launches);
+/* Kernel symbol behind the launches of kind `kind` (as above) on the
+ * plan's current schedule, "" when it has none of that kind. */
+const char* tops_plan_kernel_name(const tops_plan* plan, int kind);
 void tops_plan_destroy(tops_plan* plan);
 
 /* ESDAndCoh (Func.cpp:2496-2763) over one subswath held in HBM, with no

@@ -4,7 +4,8 @@ C-ABI (include/tops_abi.h).
 
 The compute lives in libtops_mi355x.so (HIP, gfx950); this package holds its
 ctypes bindings (native), the synthetic scenario generator (synth) and the
-device-side helpers used by the tests and the benchmark (device).
+stage driver over subswaths in host or device memory (pipeline, imported on
+demand: it needs torch for device buffers).
 """
 from . import native, synth  # noqa: F401
 

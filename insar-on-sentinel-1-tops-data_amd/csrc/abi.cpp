@@ -107,6 +107,25 @@ static int make_orbit(const tops_orbit_geom* g, OrbitDev& o)
 
 static int box_ok(const int b[4]) { return b[1] >= b[0] && b[3] >= b[2]; }
 
+// Boxes the resample kernels can take: non-empty, slave columns inside the
+// per-range arrays, slave lines within +-kMaxSlaveLine (the walker packs
+// (line, LUT row) keys into one int).
+int check_burst_geom(const tops_burst_geom* g, const char* who)
+{
+    if (!box_ok(g->master_box) || !box_ok(g->slave_box))
+        return set_error(TOPS_ERR_ARG, "%s: empty master or slave box", who);
+    if (g->slave_box[0] < 0 || g->slave_box[1] >= g->samples_per_burst)
+        return set_error(TOPS_ERR_ARG, "%s: slave columns [%d,%d] outside the %d per-range samples",
+                         who, g->slave_box[0], g->slave_box[1], g->samples_per_burst);
+    if (g->slave_box[2] <= -kMaxSlaveLine || g->slave_box[3] >= kMaxSlaveLine)
+        return set_error(TOPS_ERR_ARG, "%s: slave lines [%d,%d] outside +-%d", who,
+                         g->slave_box[2], g->slave_box[3], kMaxSlaveLine);
+    if (g->master_box[2] <= -kMaxSlaveLine || g->master_box[3] >= kMaxSlaveLine)
+        return set_error(TOPS_ERR_ARG, "%s: master lines [%d,%d] outside +-%d", who,
+                         g->master_box[2], g->master_box[3], kMaxSlaveLine);
+    return TOPS_OK;
+}
+
 }  // namespace tops
 
 using namespace tops;
@@ -191,12 +210,7 @@ int tops_resample_burst(tops_ctx* ctx, const tops_burst_geom* g, const tops_ci16
 {
     if (!ctx || !g || !d_slave || !d_kt || !d_tref || !d_fdc || !d_out)
         return set_error(TOPS_ERR_ARG, "tops_resample_burst: null argument");
-    if (!box_ok(g->master_box) || !box_ok(g->slave_box))
-        return set_error(TOPS_ERR_ARG, "tops_resample_burst: empty master or slave box");
-    if (g->slave_box[0] < 0 || g->slave_box[1] >= g->samples_per_burst)
-        return set_error(TOPS_ERR_ARG,
-                         "tops_resample_burst: slave columns [%d,%d] outside the %d per-range samples",
-                         g->slave_box[0], g->slave_box[1], g->samples_per_burst);
+    if (int rc = check_burst_geom(g, "tops_resample_burst")) return rc;
     if (mode != TOPS_MODE_FAST && mode != TOPS_MODE_FP64)
         return set_error(TOPS_ERR_ARG, "tops_resample_burst: unknown mode %d", mode);
     ResampleParams p;

@@ -7,9 +7,11 @@
 //   coh = |sum M conj(S)| / sqrt(sum|S|^2 * sum|M|^2)  (0 when the product
 //   is not positive), and coh = 0 where |S(r, c)| < 0.001.
 //
-// Fast form: a 16x64 output tile stages the products (M conj S, |S|^2,
-// |M|^2) of its (16+winAz-1) x (64+winRg-1) window in LDS, forms
-// compensated (Kahan) FP32 range sums, then short azimuth sums.
+// Fast forms: the 3 az x 10 rg window walks down bands of columns
+// (coherence_walk.hpp); any other window stages the products (M conj S,
+// |S|^2, |M|^2) of a 16x64 output tile's (16+winAz-1) x (64+winRg-1) window
+// in LDS and forms plain FP32 range sums, then short azimuth sums (each
+// window sum is a direct sum of its own terms, tile_window.hpp).
 // FP64 form: one thread per output, direct FP64 window sums from HBM.
 #include "tops_internal.hpp"
 #include "tile_window.hpp"
@@ -145,6 +147,14 @@ __global__ void __launch_bounds__(CW_THREADS) coherence_walk_batch_kernel(const 
 bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az)
 {
     return win_rg == 10 && win_az == 3 && dx == 4 && dy == 1;
+}
+
+// the batched coherence launch serves this window (walking form, or a tile
+// whose window fits in LDS); otherwise a plan forms coherence per burst
+bool coherence_batch_fits(int dx, int dy, int win_rg, int win_az)
+{
+    return coherence_walk_fits(dx, dy, win_rg, win_az) ||
+           tile_window_launch_lds(win_rg, win_az) != 0;
 }
 
 // pair loads: w even, master 8-byte, slave 16-byte, coherence 8-byte aligned

@@ -32,6 +32,9 @@ struct tops_plan {
     CohItem* d_coh = nullptr;
     int4* d_work_coh = nullptr;
     int n_work_coh = 0;
+    // the batched coherence launch serves the window; otherwise the batched
+    // schedule forms coherence with one launch per burst
+    bool coh_batched = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -75,6 +78,8 @@ int fill_params(tops_plan* p)
     std::vector<CohItem> ci;
     std::vector<int4> wc;
     p->coh_of.assign(p->items.size(), -1);
+    p->coh_batched = coherence_batch_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2,
+                                          p->coh_win_rg, p->coh_win_az);
     for (size_t i = 0; i < p->items.size(); i++) {
         const tops_burst_item& it = p->items[i];
         if (!it.master || !it.coh) continue;
@@ -93,13 +98,14 @@ int fill_params(tops_plan* p)
         } else {
             coherence_tile_count(c.w, c.h, &nx, &ny);
         }
-        for (int y = 0; y < ny; y++)
-            for (int x = 0; x < nx; x++) wc.push_back(make_int4((int)ci.size(), x, y, tag));
+        if (p->coh_batched)
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) wc.push_back(make_int4((int)ci.size(), x, y, tag));
         ci.push_back(c);
     }
     p->h_coh = ci;
     p->n_work_coh = (int)wc.size();
-    if (!ci.empty()) {
+    if (!wc.empty()) {
         TOPS_HIP(hipMalloc(&p->d_coh, sizeof(CohItem) * ci.size()));
         TOPS_HIP(hipMemcpy(p->d_coh, ci.data(), sizeof(CohItem) * ci.size(), hipMemcpyHostToDevice));
         TOPS_HIP(hipMalloc(&p->d_work_coh, sizeof(int4) * wc.size()));
@@ -136,26 +142,79 @@ int fill_work(tops_plan* p)
     return TOPS_OK;
 }
 
-int enqueue(tops_plan* p, hipStream_t s)
-{
-    const int dx = (p->coh_win_rg - 1) / 2, dy = (p->coh_win_az - 1) / 2;
-    if (p->strip_rows <= 0) {
-        for (size_t i = 0; i < p->h_params.size(); i++) {
-            int rc = launch_resample_fast(p->h_params[i], s, 0);
-            if (rc) return rc;
-            if (p->coh_of[i] >= 0) {
-                const CohItem& c = p->h_coh[p->coh_of[i]];
-                rc = launch_coherence(c.m, c.s, c.w, c.h, dx, dy, p->coh_win_rg, p->coh_win_az,
-                                      c.coh, TOPS_MODE_FAST, s);
-                if (rc) return rc;
+// Per-launch timing of a run (tops_plan_time_launches): an event pair
+// around every launch, tagged with the launch's kind.
+struct Marks {
+    std::vector<hipEvent_t> ev;   // two per launch
+    std::vector<int> kind;
+    size_t n = 0;                 // launches marked in this run
+    ~Marks()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    int before(hipStream_t s, int k)
+    {
+        if (2 * n + 2 > ev.size()) {
+            for (int j = 0; j < 2; j++) {
+                hipEvent_t e;
+                TOPS_HIP(hipEventCreate(&e));
+                ev.push_back(e);
             }
+            kind.push_back(k);
         }
+        kind[n] = k;
+        TOPS_HIP(hipEventRecord(ev[2 * n], s));
         return TOPS_OK;
     }
-    int rc = launch_resample_batch(p->d_params, p->d_work_rs, p->n_work_rs, s);
+    int after(hipStream_t s)
+    {
+        TOPS_HIP(hipEventRecord(ev[2 * n + 1], s));
+        n++;
+        return TOPS_OK;
+    }
+};
+
+enum { kKindResample = 0, kKindCoherence = 1, kKindFused = 2, kKinds = 3 };
+
+// one launch, with its event pair when the run is timed
+template <class F>
+int marked(Marks* mk, hipStream_t s, int kind, F&& launch)
+{
+    int rc = mk ? mk->before(s, kind) : TOPS_OK;
+    if (!rc) rc = launch();
+    if (!rc && mk) rc = mk->after(s);
+    return rc;
+}
+
+int enqueue(tops_plan* p, hipStream_t s, Marks* mk = nullptr)
+{
+    const int dx = (p->coh_win_rg - 1) / 2, dy = (p->coh_win_az - 1) / 2;
+    auto coh_one = [&](const CohItem& c) {
+        return marked(mk, s, kKindCoherence, [&] {
+            return launch_coherence(c.m, c.s, c.w, c.h, dx, dy, p->coh_win_rg, p->coh_win_az,
+                                    c.coh, TOPS_MODE_FAST, s);
+        });
+    };
+    int rc = TOPS_OK;
+    if (p->strip_rows <= 0) {
+        for (size_t i = 0; i < p->h_params.size() && !rc; i++) {
+            rc = marked(mk, s, kKindResample,
+                        [&] { return launch_resample_fast(p->h_params[i], s, 0); });
+            if (!rc && p->coh_of[i] >= 0) rc = coh_one(p->h_coh[p->coh_of[i]]);
+        }
+        return rc;
+    }
+    rc = marked(mk, s, kKindResample,
+                [&] { return launch_resample_batch(p->d_params, p->d_work_rs, p->n_work_rs, s); });
     if (rc) return rc;
-    return launch_coherence_batch(p->d_coh, p->d_work_coh, p->n_work_coh, dx, dy, p->coh_win_rg,
-                                  p->coh_win_az, s);
+    if (p->coh_batched)
+        return marked(mk, s, kKindCoherence, [&] {
+            return launch_coherence_batch(p->d_coh, p->d_work_coh, p->n_work_coh, dx, dy,
+                                          p->coh_win_rg, p->coh_win_az, s);
+        });
+    for (const CohItem& c : p->h_coh)
+        if ((rc = coh_one(c))) return rc;
+    return TOPS_OK;
 }
 
 void drop_graph(tops_plan* p)
@@ -180,13 +239,11 @@ tops_plan* tops_plan_create(tops_ctx* ctx, const tops_burst_item* items, int n_i
     for (int i = 0; i < n_items; i++) {
         const tops_burst_item& it = items[i];
         const tops_burst_geom& g = it.geom;
-        if (!it.slave || !it.kt || !it.tref || !it.fdc || !it.out ||
-            g.master_box[1] < g.master_box[0] || g.master_box[3] < g.master_box[2] ||
-            g.slave_box[1] < g.slave_box[0] || g.slave_box[3] < g.slave_box[2] ||
-            g.slave_box[0] < 0 || g.slave_box[1] >= g.samples_per_burst) {
-            set_error(TOPS_ERR_ARG, "tops_plan_create: burst %d is malformed", i);
+        if (!it.slave || !it.kt || !it.tref || !it.fdc || !it.out) {
+            set_error(TOPS_ERR_ARG, "tops_plan_create: burst %d has a null buffer", i);
             return nullptr;
         }
+        if (check_burst_geom(&g, "tops_plan_create")) return nullptr;
     }
     tops_plan* p = new tops_plan();
     p->ctx = ctx;
@@ -245,23 +302,83 @@ int tops_plan_capture_graph(tops_plan* p, void* stream)
 
 int tops_plan_strip_rows(const tops_plan* p) { return p ? p->strip_rows : 0; }
 
+// Launch durations of the plan's schedule: `reps` eager runs on `stream`,
+// an event pair around every launch; ms[k] = the mean per run of the
+// summed durations of the launches of kind k (0 resample, 1 coherence,
+// 2 fused resample + coherence), and launches[k] their count per run.
+int tops_plan_time_launches(tops_plan* p, void* stream, int reps, float* ms, int* launches)
+{
+    if (!p || reps <= 0 || !ms) return set_error(TOPS_ERR_ARG, "tops_plan_time_launches: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    double tot[kKinds] = {0.0, 0.0, 0.0};
+    int cnt[kKinds] = {0, 0, 0};
+    Marks mk;
+    for (int r = 0; r < reps; r++) {
+        mk.n = 0;
+        if (int rc = enqueue(p, s, &mk)) return rc;
+        TOPS_HIP(hipStreamSynchronize(s));
+        for (size_t i = 0; i < mk.n; i++) {
+            float t = 0.f;
+            TOPS_HIP(hipEventElapsedTime(&t, mk.ev[2 * i], mk.ev[2 * i + 1]));
+            tot[mk.kind[i]] += t;
+            if (r == 0) cnt[mk.kind[i]]++;
+        }
+    }
+    for (int k = 0; k < kKinds; k++) {
+        ms[k] = (float)(tot[k] / reps);
+        if (launches) launches[k] = cnt[k];
+    }
+    return TOPS_OK;
+}
+
+// Kernel behind launches of kind `kind` on the plan's current schedule
+// (the name rocprofv3 lists), or "" when the schedule has none of that kind.
+const char* tops_plan_kernel_name(const tops_plan* p, int kind)
+{
+    if (!p || kind < 0 || kind >= kKinds) return "";
+    const bool walk = coherence_walk_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2,
+                                          p->coh_win_rg, p->coh_win_az);
+    if (kind == kKindFused) return "";
+    if (p->strip_rows <= 0 || !p->coh_batched) {
+        if (kind == kKindResample) return p->strip_rows <= 0 ? "resample_fast_kernel"
+                                                             : "resample_batch_kernel";
+        if (!coherence_batch_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2,
+                                  p->coh_win_rg, p->coh_win_az))
+            return "coherence_fp64_kernel";
+        return walk ? "coherence_walk_kernel" : "coherence_fast_kernel";
+    }
+    if (kind == kKindResample) return "resample_batch_kernel";
+    return walk ? "coherence_walk_batch_kernel" : "coherence_batch_kernel";
+}
+
 // Autotuner: times every candidate schedule (per-burst launches, or one
 // batched launch at each candidate strip height) on the plan's bursts,
 // resample and coherence together (warm-up + 3 timed runs each, hipEvents
-// on `stream`), and keeps the fastest.  With TOPS_TUNE_CACHE set, the choice is read from / appended to
-// that text file ("<key> <strip_rows>" lines, key = the burst shapes).
+// on `stream`), and keeps the fastest.  With TOPS_TUNE_CACHE set, the
+// choice is read from / appended to that text file ("<key> <schedule>"
+// lines; the key names the burst count and coherence window and hashes the
+// burst shapes).
 int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
 {
     if (!p) return set_error(TOPS_ERR_ARG, "tops_plan_tune: null plan");
     hipStream_t s = (hipStream_t)stream;
-    std::string key = "resample";
+    // key: the burst count, the coherence window and a 64-bit FNV-1a hash of
+    // every burst's output shape, in order
+    uint64_t hsh = 1469598103934665603ull;
+    auto mix = [&](int v) {
+        for (int k = 0; k < 4; k++) {
+            hsh ^= (uint64_t)((v >> (8 * k)) & 0xff);
+            hsh *= 1099511628211ull;
+        }
+    };
     for (auto& it : p->items) {
-        char b[64];
-        snprintf(b, sizeof(b), "_%dx%d", it.geom.master_box[3] - it.geom.master_box[2] + 1,
-                 it.geom.master_box[1] - it.geom.master_box[0] + 1);
-        key += b;
-        if (key.size() > 400) break;
+        mix(it.geom.master_box[3] - it.geom.master_box[2] + 1);
+        mix(it.geom.master_box[1] - it.geom.master_box[0] + 1);
     }
+    char kb[96];
+    snprintf(kb, sizeof(kb), "plan_n%zu_w%dx%d_%016llx", p->items.size(), p->coh_win_az,
+             p->coh_win_rg, (unsigned long long)hsh);
+    const std::string key = kb;
     const char* cache = getenv("TOPS_TUNE_CACHE");
     if (cache) {
         FILE* f = fopen(cache, "r");
@@ -269,7 +386,7 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
             char k[512];
             int v;
             while (fscanf(f, "%511s %d", k, &v) == 2)
-                if (key == k && v > 0) {
+                if (key == k && v >= 0) {
                     fclose(f);
                     drop_graph(p);
                     p->strip_rows = v;

@@ -396,7 +396,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     int wbase = INT_MIN;   // absolute slave line held in win[0]
 
     for (int r = r0; r < r1; r++) {
-        // ---- row-uniform position terms, 64 rows at a time (one per lane)
+        // ---- row-uniform position terms, 32 rows at a time (one per lane)
         const int rr = (r - r0) & 31;
         if (rr == 0) {
             __syncthreads();
@@ -453,10 +453,11 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
         const int kleft = __shfl_up(key, 1, 64);
         const unsigned long long heads = __ballot(key != INT_MAX && (lane == 0 || kleft != key));
         if (heads != 0ull) {
-            // keys are iy * 2048 + fy with iy >= 0 (interior rows of the slave box)
+            // keys are iy * 2048 + fy (|iy| < 2^20, checked at launch; the
+            // arithmetic shift gives iy also for negative slave lines)
             int iy_min = INT_MAX;
             for (unsigned long long h = heads; h; h &= h - 1)
-                iy_min = min(iy_min, (int)((unsigned)__builtin_amdgcn_readlane(key, __builtin_ctzll(h)) >> 11));
+                iy_min = min(iy_min, __builtin_amdgcn_readlane(key, __builtin_ctzll(h)) >> 11);
             // window: rows iy_min-5 .. iy_min+7 must be held
             int o_win = wbase == INT_MIN ? -1 : iy_min - kHalf - wbase;
             if (o_win < 0 || o_win > 64) {
@@ -493,7 +494,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
                 const double abc2 = sm.abc[2][lane];
                 for (; h && nk < WK_KEYS; h &= h - 1) {
                     const int kq = __builtin_amdgcn_readlane(key, __builtin_ctzll(h));
-                    const int iyq = (int)((unsigned)kq >> 11), fyq = kq & (kLutRows - 1);
+                    const int iyq = kq >> 11, fyq = kq & (kLutRows - 1);
                     const int oq = o_win + (iyq - iy_min);
                     if (oq > 4) continue;
                     // the key's 12 taps placed at window rows oq .. oq+11 (zero

@@ -54,6 +54,10 @@ int check_desc(const tops_subswath_desc* d, const char* who)
         return set_error(TOPS_ERR_ARG, "%s: bad subswath description", who);
     if (d->n_bursts - 1 > 64)
         return set_error(TOPS_ERR_ARG, "%s: at most 65 bursts per subswath", who);
+    if ((long)d->n_bursts * d->m_lines_per_burst >= kMaxSlaveLine ||
+        (long)d->n_bursts * d->s_lines_per_burst >= kMaxSlaveLine)
+        return set_error(TOPS_ERR_ARG, "%s: more than %d lines in the subswath", who,
+                         kMaxSlaveLine - 1);
     for (int i = 0; i + 1 < d->n_bursts; i++)
         if (d->overlap_lines[i] < 0 || d->overlap_lines[i] > d->m_lines_per_burst)
             return set_error(TOPS_ERR_ARG, "%s: overlap %d has %d lines", who, i,

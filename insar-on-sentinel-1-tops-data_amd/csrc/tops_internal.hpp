@@ -21,10 +21,14 @@ constexpr int kAzShiftTaps = 16; // taps per shifted azimuth row (the window hei
 constexpr int kInterval = 2047;
 constexpr int kTaps = 12;                         // Main.cpp:75, 113
 constexpr int kHalf = kTaps / 2 - 1;              // c_Npoints2m1
+// slave (and master) lines a burst geometry may address: the resample walker
+// packs (line, LUT row) into one int as line * 2048 + row
+constexpr int kMaxSlaveLine = (1 << 20) - 64;
 
 // ---- status (thread-local, read through tops_last_error) ---------------
 int set_error(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
+int check_burst_geom(const tops_burst_geom* g, const char* who);
 
 #define TOPS_HIP(call)                                                       \
     do {                                                                     \
@@ -136,6 +140,7 @@ int coherence_tile_count(int w, int h, int* nx, int* ny);
 // walking form (3 az x 10 rg window, dx = 4, dy = 1); batched work items are
 // {item, column band, row block, coherence_walk_vec(...)}
 bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az);
+bool coherence_batch_fits(int dx, int dy, int win_rg, int win_az);
 int coherence_walk_vec(const void* m, const void* s, const void* coh, int w);
 int coherence_walk_count(int w, int h, int* nx, int* ny);
 

@@ -248,6 +248,7 @@ class Pair:
     dem_spacing: float               # arc-seconds
     speed: float = 7500.0
     coh_win: tuple = (3, 10)         # (az, rg)
+    noise_stream: int = None         # phase-noise stream of the slave (default stream + 1000)
 
     def params(self, which="slave"):
         sw = self.slave if which == "slave" else self.master
@@ -278,10 +279,11 @@ class Pair:
         else:
             az = np.ascontiguousarray(self.az_off, dtype=np.float64)
             rg = np.ascontiguousarray(self.rg_off, dtype=np.float64)
+            noise = self.stream + 1000 if self.noise_stream is None else self.noise_stream
             S.tsyn_burst(self.seed, self.stream, az_band, rg_band, 300.0, int(sw.scene_line0[b]),
                          L, P, native.ptr(az), native.ptr(rg), self.delta, sw.az_time_interval,
                          native.ptr(kt_b), native.ptr(tr_b), native.ptr(fd_b),
-                         self.phase_noise, self.stream + 1000, out.ctypes.data)
+                         self.phase_noise, noise, out.ctypes.data)
         return out
 
     # --- DEM --------------------------------------------------------------
@@ -399,3 +401,83 @@ def subswath_iw1(seed=20260103, n_bursts=9, lines=1500, pixels=21000, delta=0.00
 def orbit_polyfit_inputs(orbit, n=10, spacing=10.0):
     t, p, v = orbit.state_vectors(n, spacing)
     return t, p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy()
+
+
+# ---------------------------------------------------------------------------
+# batched workloads: many bursts of one plan
+# ---------------------------------------------------------------------------
+@dataclass
+class BatchBurst:
+    """One burst of a batched workload: burst `index` of `pair`, with its
+    master box on the master grid (absolute lines y0 .. y0 + lines - 1)."""
+    pair: Pair
+    index: int
+    group: int       # subswath (IW pair) or slave (stack) it belongs to
+    lines: int
+    width: int
+    y0: int
+
+    @property
+    def box(self):
+        return [0, self.width - 1, self.y0, self.y0 + self.lines - 1]
+
+    @property
+    def dt(self):
+        return self.pair.master.az_time_interval
+
+    def cpm(self):
+        return self.pair.cpm(self.index)
+
+    def master(self):
+        return self.pair.burst(self.index, "master")
+
+    def slave(self):
+        return self.pair.burst(self.index, "slave")
+
+    def params(self):
+        """kt, tref, fdc of the slave burst (contiguous [width] arrays)"""
+        kt, tref, fdc, _ = self.pair.params()
+        return tuple(np.ascontiguousarray(a[self.index]) for a in (kt, tref, fdc))
+
+
+def iw_pair_bursts(seed=2026, widths=(21000, 25000, 24500), n_bursts=9, lines=(1480, 1520)):
+    """The batched IW pair: 3 subswaths x 9 bursts of 21000 / 25000 / 24500
+    samples with line counts drawn uniformly from 1480-1520.  Every burst is
+    its own seeded scene with its own offset field (az 0-0.3 px, rg 0.5-2
+    px, bilinear) and Doppler centroid slope (10-40 Hz over the swath)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for sw, width in enumerate(widths):
+        counts = rng.integers(lines[0], lines[1] + 1, size=n_bursts)
+        for b in range(n_bursts):
+            amp_az, amp_rg = rng.uniform(0.0, 0.3), rng.uniform(0.5, 2.0)
+            az = np.array([rng.uniform(0.0, 0.3 - 0.5 * amp_az) + 0.0, amp_az * 0.5,
+                           amp_az * 0.3, 0.0, amp_az * 0.2, 0.0])
+            rg = np.array([0.5, (amp_rg - 0.5) * 0.5, (amp_rg - 0.5) * 0.3, 0.0,
+                           (amp_rg - 0.5) * 0.2, 0.0])
+            nl = int(counts[b])
+            pair = make_pair(f"iw{sw + 1}_b{b}", nl, width, seed=seed * 1000 + sw * 100 + b,
+                             az_off=az, rg_off=rg, dem_spacing=1.0,
+                             dc_slope_total=float(rng.uniform(10.0, 40.0)))
+            out.append(BatchBurst(pair, 0, sw, nl, width, 0))
+    return out
+
+
+def stack_bursts(seed=4100, n_slaves=16, n_bursts=3, lines=1500, width=25000):
+    """The stack: one IW2 master of 3 bursts (1500 x 25000, 100-line
+    overlaps) against 16 slaves that image the same scene, each with its own
+    offset field (az 0-0.5 px, rg 0-3 px) and Doppler centroid (0-60 Hz) and
+    its own phase noise.  Bursts are listed slave by slave."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_slaves):
+        a, r = rng.uniform(0.0, 0.25, size=4), rng.uniform(0.0, 1.5, size=4)
+        az = np.array([a[0], a[1] * 0.5, a[2] * 0.3, 0.0, a[3] * 0.2, 0.0])
+        rg = np.array([r[0], r[1] * 0.5, r[2] * 0.3, 0.0, r[3] * 0.2, 0.0])
+        pair = make_pair("stack_iw2", lines, width, n_bursts=n_bursts, overlap=100, seed=seed,
+                         stream=0, az_off=az, rg_off=rg, dem_spacing=1.0, dc_slope_total=10.0,
+                         slave_dc0=float(rng.uniform(0.0, 60.0)))
+        pair.noise_stream = 1000 + k
+        for b in range(n_bursts):
+            out.append(BatchBurst(pair, b, k, lines, width, b * lines))
+    return out

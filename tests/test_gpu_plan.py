@@ -173,3 +173,105 @@ def test_plan_per_burst_graph_replay(tops, dev, batch):
             assert np.array_equal(dev.get(d["out"], np.complex64, (b["L"], b["P"])), eo), rep
             assert np.array_equal(dev.get(d["coh"], np.float32, (b["L"], b["P"])), ec), rep
     L.tops_plan_destroy(plan)
+
+
+def test_plan_reference_window_6x24(tops, orc, dev, batch):
+    """The reference's own ESDAndCoh window (6 az x 24 rg, dx = 11, dy = 2,
+    Func.cpp:2583-2584, 2643-2644) through the batched schedule (the tiled
+    coherence kernel) and its hipGraph replay, against the oracle."""
+    N = tops.native
+    L = N.load()
+    torch = dev.torch
+    plan = L.tops_plan_create(batch["ctx"], batch["items"], 5, 6, 24, 64)
+    assert plan, N.last_error()
+    s = torch.cuda.Stream()
+    sp = C.c_void_p(s.cuda_stream)
+    N.check(L.tops_plan_run(plan, sp))
+    s.synchronize()
+    eager = []
+    for i, (b, d) in enumerate(zip(batch["bursts"], batch["dev"])):
+        got = dev.get(d["out"], np.complex64, (b["L"], b["P"])).copy()
+        coh = dev.get(d["coh"], np.float32, (b["L"], b["P"])).copy()
+        cref = orc.coherence(b["M"], got.astype(np.complex128), 11, 2, 24, 6)
+        ec = float(np.max(np.abs(coh - cref)))
+        print(f"  burst {i}: coherence 6x24 {ec:.2e}")
+        assert ec <= 1e-4
+        eager.append(coh)
+    N.check(L.tops_plan_capture_graph(plan, sp), "tops_plan_capture_graph")
+    for rep in range(2):
+        for d in batch["dev"]:
+            d["coh"].zero_()
+        torch.cuda.synchronize()
+        N.check(L.tops_plan_run(plan, sp))
+        s.synchronize()
+        for (b, d), ec in zip(zip(batch["bursts"], batch["dev"]), eager):
+            assert np.array_equal(dev.get(d["coh"], np.float32, (b["L"], b["P"])), ec), rep
+    L.tops_plan_destroy(plan)
+
+
+def test_plan_window_beyond_lds_tile(tops, orc, dev, batch):
+    """A window whose tile does not fit in LDS (21 x 81): the plan is
+    accepted and forms coherence per burst (the FP64 kernel), also on the
+    batched schedule, instead of failing at run or capture time."""
+    N = tops.native
+    L = N.load()
+    plan = L.tops_plan_create(batch["ctx"], batch["items"], 5, 21, 81, 64)
+    assert plan, N.last_error()
+    N.check(L.tops_plan_run(plan, None), "tops_plan_run")
+    b, d = batch["bursts"][0], batch["dev"][0]
+    got = dev.get(d["out"], np.complex64, (b["L"], b["P"])).copy()
+    coh = dev.get(d["coh"], np.float32, (b["L"], b["P"])).copy()
+    cref = orc.coherence(b["M"], got.astype(np.complex128), 40, 10, 81, 21)
+    ec = float(np.max(np.abs(coh - cref)))
+    print(f"  coherence 21x81: {ec:.2e}")
+    assert ec <= 1e-4
+    L.tops_plan_destroy(plan)
+
+
+def test_plan_shifted_slave_origin(tops, orc, dev):
+    """Slave boxes whose first line is negative or far down the raster (the
+    walker's (line, LUT row) keys then decode with the line's sign): both
+    the single-burst path and a plan agree with the oracle; lines beyond
+    2^20 are refused."""
+    N = tops.native
+    L = N.load()
+    pair = tops.synth.make_pair("origin", 96, 300, seed=11)
+    m = pair.master
+    kaz, krg = N.lut_init(12, m.prf, m.azimuth_bandwidth, 2 * m.range_sampling_rate)
+    kt, tref, fdc, _ = pair.params()
+    S = pair.burst(0, "slave")
+    ctx = L.tops_ctx_create(N.ptr(kaz, C.c_float), N.ptr(krg, C.c_float), 12)
+    dS, dk, dtr, dfd = (dev.put(a) for a in (S, kt[0], tref[0], fdc[0]))
+    for y0 in (-40, 700000):
+        ca, cr = pair.cpm(0)
+        ca = ca.copy()
+        ca[0] += y0
+        mbox = [0, 299, 0, 95]
+        sbox = [0, 299, y0, y0 + 95]
+        g = N.BurstGeom((C.c_int * 4)(*mbox), (C.c_int * 4)(*sbox), y0, 300, m.az_time_interval,
+                        0.0, (C.c_double * 6)(*ca), (C.c_double * 6)(*cr))
+        out = dev.empty(96 * 300 * 8)
+        N.check(L.tops_resample_burst(ctx, C.byref(g), dS.data_ptr(), dk.data_ptr(),
+                                      dtr.data_ptr(), dfd.data_ptr(), out.data_ptr(), 0, None))
+        got = dev.get(out, np.complex64, (96, 300)).copy()
+        ref = orc.resample(S, mbox, sbox, y0, m.az_time_interval, 0.0, ca, cr, kaz, krg, kt[0],
+                           tref[0], fdc[0])
+        e = rel_err(got, ref)
+        item = (N.BurstItem * 1)()
+        out2 = dev.empty(96 * 300 * 8)
+        item[0] = N.BurstItem(g, dS.data_ptr(), dk.data_ptr(), dtr.data_ptr(), dfd.data_ptr(),
+                              out2.data_ptr(), None, None)
+        plan = L.tops_plan_create(ctx, item, 1, 3, 10, 32)
+        assert plan, N.last_error()
+        N.check(L.tops_plan_run(plan, None))
+        got2 = dev.get(out2, np.complex64, (96, 300))
+        L.tops_plan_destroy(plan)
+        e2 = rel_err(got2, ref)
+        print(f"  slave origin {y0}: single {e:.2e}, plan {e2:.2e}, interior "
+              f"{int(np.count_nonzero(ref))} of {ref.size}")
+        assert e <= 1e-5 and e2 <= 1e-5 and np.count_nonzero(ref) > ref.size // 2
+    g.slave_box[2], g.slave_box[3] = 1 << 20, (1 << 20) + 95
+    assert L.tops_resample_burst(ctx, C.byref(g), dS.data_ptr(), dk.data_ptr(), dtr.data_ptr(),
+                                 dfd.data_ptr(), out.data_ptr(), 0, None) == N.TOPS_ERR_ARG
+    L.tops_clear_error()
+    L.tops_ctx_destroy(ctx)
