@@ -108,8 +108,10 @@ static int make_orbit(const tops_orbit_geom* g, OrbitDev& o)
 static int box_ok(const int b[4]) { return b[1] >= b[0] && b[3] >= b[2]; }
 
 // Boxes the resample kernels can take: non-empty, slave columns inside the
-// per-range arrays, slave lines within +-kMaxSlaveLine (the walker packs
-// (line, LUT row) keys into one int).
+// per-range arrays, slave lines in [0, kMaxSlaveLine).  GetIndexes
+// (DerampAndResample.cu:108-112) truncates toward zero, so a negative
+// interior position would give a negative LUT row; and the walker packs
+// (line, LUT row) keys into one int.
 int check_burst_geom(const tops_burst_geom* g, const char* who)
 {
     if (!box_ok(g->master_box) || !box_ok(g->slave_box))
@@ -117,8 +119,8 @@ int check_burst_geom(const tops_burst_geom* g, const char* who)
     if (g->slave_box[0] < 0 || g->slave_box[1] >= g->samples_per_burst)
         return set_error(TOPS_ERR_ARG, "%s: slave columns [%d,%d] outside the %d per-range samples",
                          who, g->slave_box[0], g->slave_box[1], g->samples_per_burst);
-    if (g->slave_box[2] <= -kMaxSlaveLine || g->slave_box[3] >= kMaxSlaveLine)
-        return set_error(TOPS_ERR_ARG, "%s: slave lines [%d,%d] outside +-%d", who,
+    if (g->slave_box[2] < 0 || g->slave_box[3] >= kMaxSlaveLine)
+        return set_error(TOPS_ERR_ARG, "%s: slave lines [%d,%d] outside [0, %d)", who,
                          g->slave_box[2], g->slave_box[3], kMaxSlaveLine);
     if (g->master_box[2] <= -kMaxSlaveLine || g->master_box[3] >= kMaxSlaveLine)
         return set_error(TOPS_ERR_ARG, "%s: master lines [%d,%d] outside +-%d", who,

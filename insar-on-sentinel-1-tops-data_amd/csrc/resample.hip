@@ -453,8 +453,8 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
         const int kleft = __shfl_up(key, 1, 64);
         const unsigned long long heads = __ballot(key != INT_MAX && (lane == 0 || kleft != key));
         if (heads != 0ull) {
-            // keys are iy * 2048 + fy (|iy| < 2^20, checked at launch; the
-            // arithmetic shift gives iy also for negative slave lines)
+            // keys are iy * 2048 + fy with 0 <= iy < 2^20 (interior rows of a
+            // slave box checked at launch) and 0 <= fy < 2048
             int iy_min = INT_MAX;
             for (unsigned long long h = heads; h; h &= h - 1)
                 iy_min = min(iy_min, __builtin_amdgcn_readlane(key, __builtin_ctzll(h)) >> 11);

@@ -21,8 +21,8 @@ constexpr int kAzShiftTaps = 16; // taps per shifted azimuth row (the window hei
 constexpr int kInterval = 2047;
 constexpr int kTaps = 12;                         // Main.cpp:75, 113
 constexpr int kHalf = kTaps / 2 - 1;              // c_Npoints2m1
-// slave (and master) lines a burst geometry may address: the resample walker
-// packs (line, LUT row) into one int as line * 2048 + row
+// slave lines a burst geometry may address (0 .. kMaxSlaveLine - 1): the
+// resample walker packs (line, LUT row) into one int as line * 2048 + row
 constexpr int kMaxSlaveLine = (1 << 20) - 64;
 
 // ---- status (thread-local, read through tops_last_error) ---------------

@@ -229,10 +229,11 @@ def test_plan_window_beyond_lds_tile(tops, orc, dev, batch):
 
 
 def test_plan_shifted_slave_origin(tops, orc, dev):
-    """Slave boxes whose first line is negative or far down the raster (the
-    walker's (line, LUT row) keys then decode with the line's sign): both
-    the single-burst path and a plan agree with the oracle; lines beyond
-    2^20 are refused."""
+    """A slave box far down the raster (first line 700000: the walker's
+    packed (line, LUT row) keys near the top of their range): the single-
+    burst path and a plan agree with the oracle.  Negative slave lines
+    (GetIndexes truncates toward zero, so an interior position there would
+    take a negative LUT row) and lines beyond 2^20 are refused by both."""
     N = tops.native
     L = N.load()
     pair = tops.synth.make_pair("origin", 96, 300, seed=11)
@@ -242,7 +243,7 @@ def test_plan_shifted_slave_origin(tops, orc, dev):
     S = pair.burst(0, "slave")
     ctx = L.tops_ctx_create(N.ptr(kaz, C.c_float), N.ptr(krg, C.c_float), 12)
     dS, dk, dtr, dfd = (dev.put(a) for a in (S, kt[0], tref[0], fdc[0]))
-    for y0 in (-40, 700000):
+    for y0 in (700000,):
         ca, cr = pair.cpm(0)
         ca = ca.copy()
         ca[0] += y0
@@ -270,8 +271,13 @@ def test_plan_shifted_slave_origin(tops, orc, dev):
         print(f"  slave origin {y0}: single {e:.2e}, plan {e2:.2e}, interior "
               f"{int(np.count_nonzero(ref))} of {ref.size}")
         assert e <= 1e-5 and e2 <= 1e-5 and np.count_nonzero(ref) > ref.size // 2
-    g.slave_box[2], g.slave_box[3] = 1 << 20, (1 << 20) + 95
-    assert L.tops_resample_burst(ctx, C.byref(g), dS.data_ptr(), dk.data_ptr(), dtr.data_ptr(),
-                                 dfd.data_ptr(), out.data_ptr(), 0, None) == N.TOPS_ERR_ARG
-    L.tops_clear_error()
+    for lo in (-40, 1 << 20):
+        g.slave_box[2], g.slave_box[3] = lo, lo + 95
+        assert L.tops_resample_burst(ctx, C.byref(g), dS.data_ptr(), dk.data_ptr(),
+                                     dtr.data_ptr(), dfd.data_ptr(), out.data_ptr(), 0,
+                                     None) == N.TOPS_ERR_ARG
+        item[0] = N.BurstItem(g, dS.data_ptr(), dk.data_ptr(), dtr.data_ptr(), dfd.data_ptr(),
+                              out2.data_ptr(), None, None)
+        assert not L.tops_plan_create(ctx, item, 1, 3, 10, 32)
+        L.tops_clear_error()
     L.tops_ctx_destroy(ctx)
