@@ -187,6 +187,22 @@ int tops_esd_shifts(const tops_ci16* d_m_for, const tops_ci16* d_m_back,
                     double spectral_sep, double az_time_interval,
                     double* shifts, double* sums, void* stream);
 
+/* tops_esd_shifts without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate (code that runs beside HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as tops_esd_shifts; the tile partials
+ * and band sums live in a fixed per-GPU device array of 4 MiB (at most
+ * 262144 tiles of 16 lines x 64 samples, i.e. n_overlap x ceil(pixels/64) x
+ * ceil(max overlap/16), and at most 64 overlaps), held by each GPU the
+ * library runs on until the process ends.  Larger calls return TOPS_ERR_ARG.
+ * Calls that overlap on one GPU share that array and race. */
+int tops_esd_shifts_noalloc(const tops_ci16* d_m_for, const tops_ci16* d_m_back,
+                            const tops_cf32* d_s_for, const tops_cf32* d_s_back,
+                            const int* overlap_size, int n_overlap, int pixels,
+                            int win_az, int win_rg, float threshold,
+                            double spectral_sep, double az_time_interval,
+                            double* shifts, double* sums, void* stream);
+
 /* ---- batched bursts (no counterpart in the reference, which runs one
  * burst per call with a device reset in between) ---------------------- */
 typedef struct {

@@ -26,6 +26,7 @@
 #include "tops_internal.hpp"
 #include "tile_window.hpp"
 
+#include <mutex>
 #include <vector>
 
 namespace tops {
@@ -264,4 +265,93 @@ int launch_esd_bands(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* 
     return TOPS_OK;
 }
 
+// ---- allocation-free ESD -------------------------------------------------
+// The tile partials and the band sums of tops_esd_shifts_noalloc live in one
+// fixed device array.  Every GPU the library runs on holds its own copy until
+// the process ends; its address on each device is looked up once, with that
+// device current, and kept here.  The host name `scratch` is only a stand-in
+// for the symbol: kernels and copies take the looked-up address.
+constexpr int kEsdNoAllocTiles = 262144;   // tile partials: 4 MiB of double2
+static __device__ __align__(256) double2 scratch[kEsdNoAllocTiles + kMaxEsdBands];
+
+constexpr int kMaxDevices = 64;
+static double2* g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int esd_scratch_ptr(double2** out)
+{
+    int dev = -1;
+    TOPS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(TOPS_ERR_NODEVICE, "tops_esd_shifts_noalloc: device %d beyond %d", dev,
+                         kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        TOPS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = (double2*)p;
+    }
+    *out = g_scratch_dev[dev];
+    return TOPS_OK;
+}
+
 }  // namespace tops
+
+using namespace tops;
+
+extern "C" int tops_esd_shifts_noalloc(const tops_ci16* d_m_for, const tops_ci16* d_m_back,
+                                       const tops_cf32* d_s_for, const tops_cf32* d_s_back,
+                                       const int* overlap_size, int n_overlap, int pixels,
+                                       int win_az, int win_rg, float threshold,
+                                       double spectral_sep, double az_time_interval,
+                                       double* shifts, double* sums, void* stream)
+{
+    // every check comes before the first HIP call
+    if (n_overlap < 0 || (n_overlap > 0 && (!overlap_size || !shifts || pixels <= 0)))
+        return set_error(TOPS_ERR_ARG, "tops_esd_shifts_noalloc: bad arguments");
+    if (n_overlap == 0) return TOPS_OK;
+    if (n_overlap > kMaxEsdBands)
+        return set_error(TOPS_ERR_ARG, "tops_esd_shifts_noalloc: %d overlaps exceed the limit of %d",
+                         n_overlap, kMaxEsdBands);
+    for (int i = 0; i < n_overlap; i++)
+        if (overlap_size[i] < 0)
+            return set_error(TOPS_ERR_ARG, "tops_esd_shifts_noalloc: negative overlap");
+    int maxn = 0;
+    for (int i = 0; i < n_overlap; i++) maxn = overlap_size[i] > maxn ? overlap_size[i] : maxn;
+    // the launch's grid (esd_partial_capacity), counted without int overflow
+    const long tiles = (long)n_overlap * ((pixels + TW_COLS - 1L) / TW_COLS) *
+                       (maxn > 0 ? (maxn + TW_ROWS - 1L) / TW_ROWS : 1L);
+    if (tiles > kEsdNoAllocTiles)
+        return set_error(TOPS_ERR_ARG,
+                         "tops_esd_shifts_noalloc: %ld tiles of 16 x 64 exceed the fixed scratch "
+                         "of %d tiles (use tops_esd_shifts)", tiles, kEsdNoAllocTiles);
+    if (tile_window_launch_lds(win_rg, win_az) == 0)
+        return set_error(TOPS_ERR_ARG, "tops_esd_shifts_noalloc: window %dx%d exceeds the LDS tile",
+                         win_az, win_rg);
+    double2* base = nullptr;
+    int rc = esd_scratch_ptr(&base);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    // bands concatenated in order in all four arrays (EstESDShifts layout);
+    // every tile writes its partial and every band its sum, so nothing in the
+    // array needs clearing between calls
+    int off[kMaxEsdBands];
+    off[0] = 0;
+    for (int i = 1; i < n_overlap; i++) off[i] = off[i - 1] + overlap_size[i - 1];
+    double* d_sums = (double*)(base + kEsdNoAllocTiles);
+    rc = launch_esd_bands(d_m_for, d_m_back, d_s_for, d_s_back, off, off, overlap_size, n_overlap,
+                          pixels, win_az, win_rg, threshold, d_sums, (double*)base, s);
+    if (rc) return rc;
+    double h[2 * kMaxEsdBands];
+    TOPS_HIP(hipMemcpyAsync(h, d_sums, sizeof(double) * 2 * n_overlap, hipMemcpyDeviceToHost, s));
+    TOPS_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n_overlap; i++) {
+        // EstESD.cu:1213-1217
+        shifts[i] = atan2(h[2 * i + 1], h[2 * i]) / (2 * kPi * spectral_sep * az_time_interval);
+        if (sums) {
+            sums[2 * i] = h[2 * i];
+            sums[2 * i + 1] = h[2 * i + 1];
+        }
+    }
+    return TOPS_OK;
+}

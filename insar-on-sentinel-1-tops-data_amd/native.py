@@ -29,7 +29,7 @@ EXPORTS = [
     "GeoCoreg_warp_cuda", "DerampDemodResample", "DerampDemodResample_ESD",
     "ResampleFirstBurst", "EstESDShifts", "CoherenceEst", "CoherenceEst_ESD",
     "tops_ctx_create", "tops_ctx_destroy", "tops_resample_burst",
-    "tops_coherence", "tops_esd_shifts", "tops_geocoreg",
+    "tops_coherence", "tops_esd_shifts", "tops_esd_shifts_noalloc", "tops_geocoreg",
     "tops_lut_init", "tops_polyfit_orbit", "tops_orbit_speed", "tops_deramp_params",
     "tops_overlap_sizes", "tops_spectral_sep", "tops_geometric_coreg",
     "tops_deramp_and_resample", "tops_esd_and_coh", "tops_read_config",
@@ -185,6 +185,8 @@ def load():
     L.tops_esd_shifts.restype = C.c_int
     L.tops_esd_shifts.argtypes = [_vp, _vp, _vp, _vp, _ip, C.c_int, C.c_int, C.c_int,
                                   C.c_int, C.c_float, C.c_double, C.c_double, _dp, _dp, _vp]
+    L.tops_esd_shifts_noalloc.restype = C.c_int
+    L.tops_esd_shifts_noalloc.argtypes = L.tops_esd_shifts.argtypes
     L.tops_geocoreg.restype = C.c_int
     L.tops_geocoreg.argtypes = [_vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                 C.c_double, C.c_int, C.POINTER(OrbitGeom),
