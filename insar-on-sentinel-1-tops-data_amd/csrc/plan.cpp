@@ -21,7 +21,9 @@ struct tops_plan {
     int coh_win_az = 3, coh_win_rg = 10;
     // schedule: 0 = one launch pair per burst (resample, then coherence) with
     // each burst's automatic heights; > 0 = one batched launch of all bursts'
-    // strips of this height, then one batched coherence launch
+    // strips of this height, then one batched coherence launch; < 0 = one
+    // fused persistent launch (resample and coherence) over row bands of
+    // -strip_rows lines (3 az x 10 rg window only)
     int strip_rows = 0;
     std::vector<ResampleParams> h_params;
     std::vector<CohItem> h_coh;
@@ -35,6 +37,14 @@ struct tops_plan {
     // the batched coherence launch serves the window; otherwise the batched
     // schedule forms coherence with one launch per burst
     bool coh_batched = true;
+    // fused schedule: coherence item of each burst (-1: none), the work
+    // table and the queue counters of the persistent launch
+    bool fused_ok = false;
+    CohItem* d_coh_all = nullptr;
+    int* d_coh_of = nullptr;
+    int4* d_work_fused = nullptr;
+    int n_work_fused = 0;
+    unsigned* d_queue = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -104,6 +114,21 @@ int fill_params(tops_plan* p)
         ci.push_back(c);
     }
     p->h_coh = ci;
+    p->fused_ok = fused_window_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2,
+                                    p->coh_win_rg, p->coh_win_az);
+    if (p->fused_ok) {
+        if (!ci.empty()) {
+            TOPS_HIP(hipMalloc(&p->d_coh_all, sizeof(CohItem) * ci.size()));
+            TOPS_HIP(hipMemcpy(p->d_coh_all, ci.data(), sizeof(CohItem) * ci.size(),
+                               hipMemcpyHostToDevice));
+        }
+        TOPS_HIP(hipMalloc(&p->d_coh_of, sizeof(int) * p->coh_of.size()));
+        TOPS_HIP(hipMemcpy(p->d_coh_of, p->coh_of.data(), sizeof(int) * p->coh_of.size(),
+                           hipMemcpyHostToDevice));
+        // the queue is cleared once here; every launch leaves it cleared
+        TOPS_HIP(hipMalloc(&p->d_queue, 2 * sizeof(unsigned)));
+        TOPS_HIP(hipMemset(p->d_queue, 0, 2 * sizeof(unsigned)));
+    }
     p->n_work_coh = (int)wc.size();
     if (!wc.empty()) {
         TOPS_HIP(hipMalloc(&p->d_coh, sizeof(CohItem) * ci.size()));
@@ -118,7 +143,33 @@ int fill_params(tops_plan* p)
 // interleaved row-band by row-band so uneven bursts spread over the GPU
 int fill_work(tops_plan* p)
 {
-    if (p->strip_rows <= 0) return TOPS_OK;   // per-burst launches need no table
+    if (p->strip_rows < 0) {
+        // fused: row band by row band, burst by burst, strip by strip
+        const int rows = -p->strip_rows;
+        std::vector<int4> w;
+        int max_lines = 0;
+        for (auto& it : p->items)
+            max_lines = std::max(max_lines, it.geom.master_box[3] - it.geom.master_box[2] + 1);
+        for (int y0 = 0; y0 < max_lines; y0 += rows)
+            for (size_t i = 0; i < p->items.size(); i++) {
+                const tops_burst_geom& g = p->items[i].geom;
+                const int lines = g.master_box[3] - g.master_box[2] + 1;
+                if (y0 >= lines) continue;
+                const int nx = fused_strip_count(g.master_box[1] - g.master_box[0] + 1);
+                for (int x = 0; x < nx; x++)
+                    w.push_back(make_int4((int)i, x, y0, std::min(rows, lines - y0)));
+            }
+        if (p->d_work_fused) (void)hipFree(p->d_work_fused);
+        p->d_work_fused = nullptr;
+        p->n_work_fused = (int)w.size();
+        if (!w.empty()) {
+            TOPS_HIP(hipMalloc(&p->d_work_fused, sizeof(int4) * w.size()));
+            TOPS_HIP(hipMemcpy(p->d_work_fused, w.data(), sizeof(int4) * w.size(),
+                               hipMemcpyHostToDevice));
+        }
+        return TOPS_OK;
+    }
+    if (p->strip_rows == 0) return TOPS_OK;   // per-burst launches need no table
     std::vector<int4> w;
     int max_ny = 0;
     std::vector<int> nxs(p->items.size()), nys(p->items.size());
@@ -196,7 +247,12 @@ int enqueue(tops_plan* p, hipStream_t s, Marks* mk = nullptr)
         });
     };
     int rc = TOPS_OK;
-    if (p->strip_rows <= 0) {
+    if (p->strip_rows < 0)
+        return marked(mk, s, kKindFused, [&] {
+            return launch_fused_batch(p->d_params, p->d_coh_all, p->d_coh_of, p->d_work_fused,
+                                      p->n_work_fused, p->d_queue, s);
+        });
+    if (p->strip_rows == 0) {
         for (size_t i = 0; i < p->h_params.size() && !rc; i++) {
             rc = marked(mk, s, kKindResample,
                         [&] { return launch_resample_fast(p->h_params[i], s, 0); });
@@ -250,8 +306,15 @@ tops_plan* tops_plan_create(tops_ctx* ctx, const tops_burst_item* items, int n_i
     p->items.assign(items, items + n_items);
     p->coh_win_az = coh_win_az;
     p->coh_win_rg = coh_win_rg;
-    p->strip_rows = strip_rows > 0 ? strip_rows : 0;
-    if (fill_params(p) || fill_work(p)) {
+    p->strip_rows = strip_rows;
+    if (fill_params(p)) {
+        tops_plan_destroy(p);
+        return nullptr;
+    }
+    // the fused schedule serves the 3 az x 10 rg window; other windows take
+    // the batched schedule at the same height
+    if (p->strip_rows < 0 && !p->fused_ok) p->strip_rows = -p->strip_rows;
+    if (fill_work(p)) {
         tops_plan_destroy(p);
         return nullptr;
     }
@@ -266,6 +329,10 @@ void tops_plan_destroy(tops_plan* p)
     if (p->d_work_rs) (void)hipFree(p->d_work_rs);
     if (p->d_coh) (void)hipFree(p->d_coh);
     if (p->d_work_coh) (void)hipFree(p->d_work_coh);
+    if (p->d_coh_all) (void)hipFree(p->d_coh_all);
+    if (p->d_coh_of) (void)hipFree(p->d_coh_of);
+    if (p->d_work_fused) (void)hipFree(p->d_work_fused);
+    if (p->d_queue) (void)hipFree(p->d_queue);
     delete p;
 }
 
@@ -338,6 +405,7 @@ const char* tops_plan_kernel_name(const tops_plan* p, int kind)
     if (!p || kind < 0 || kind >= kKinds) return "";
     const bool walk = coherence_walk_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2,
                                           p->coh_win_rg, p->coh_win_az);
+    if (p->strip_rows < 0) return kind == kKindFused ? "burst_fused_kernel" : "";
     if (kind == kKindFused) return "";
     if (p->strip_rows <= 0 || !p->coh_batched) {
         if (kind == kKindResample) return p->strip_rows <= 0 ? "resample_fast_kernel"
@@ -386,7 +454,7 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
             char k[512];
             int v;
             while (fscanf(f, "%511s %d", k, &v) == 2)
-                if (key == k && v >= 0) {
+                if (key == k && (v >= 0 || p->fused_ok)) {
                     fclose(f);
                     drop_graph(p);
                     p->strip_rows = v;
@@ -396,9 +464,12 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
             fclose(f);
         }
     }
-    // per-burst launches (0), fixed batched heights, and the batched heights
-    // that fill 1, 2 and 3 rounds of the GPU's walker wave slots
+    // per-burst launches (0), fixed batched heights, the batched heights
+    // that fill 1, 2 and 3 rounds of the GPU's walker wave slots, and the
+    // fused launch's row bands (negative)
     std::vector<int> cand = {0, 32, 64, 128, 192, 256, 320, 384, 512};
+    if (p->fused_ok)
+        for (int r : {64, 128, 192, 256, 384, 512, 768}) cand.push_back(-r);
     {
         std::vector<int> ls, ps;
         for (auto& it : p->items) {

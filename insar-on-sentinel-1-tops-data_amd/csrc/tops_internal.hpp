@@ -144,6 +144,17 @@ bool coherence_batch_fits(int dx, int dy, int win_rg, int win_az);
 int coherence_walk_vec(const void* m, const void* s, const void* coh, int w);
 int coherence_walk_count(int w, int h, int* nx, int* ny);
 
+// launchers (fused.hip): one persistent launch of deramp -> resample ->
+// reramp -> coherence over work items {burst, column strip, first row, rows}
+// for the 3 az x 10 rg window (dx = 4, dy = 1); coh_of[burst] indexes
+// coh_items or is -1 (no coherence); queue: 2 counters, zero before the
+// first launch (each launch leaves them zero)
+bool fused_window_fits(int dx, int dy, int win_rg, int win_az);
+int fused_strip_count(int m_pixels);
+int fused_grid();
+int launch_fused_batch(const ResampleParams* d_items, const CohItem* d_coh, const int* d_coh_of,
+                       const int4* d_work, int n_work, unsigned* d_queue, hipStream_t s);
+
 // launchers (esd.hip): writes 2*n_overlap double sums to d_sums
 int launch_esd(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* sf,
                const tops_cf32* sb, const int* overlap_size, int n_overlap,

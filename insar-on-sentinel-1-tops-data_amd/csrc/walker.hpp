@@ -1,0 +1,227 @@
+// walker.hpp -- device pieces shared by the column-walking resample kernels
+// (resample.hip: one wave per strip; fused.hip: the persistent fused
+// resample + coherence kernel): slave position and region rules
+// (DerampAndResample.cu:108-112, 348-403), the deramp phase coefficients
+// (DerampAndResample.cu:117-183), the per-lane column walker with its FP64
+// phasor recurrence, the 12-tap azimuth sums, the reramp and the direct
+// per-output path.
+#pragma once
+
+#include "tops_internal.hpp"
+
+namespace tops {
+
+// AzimuthShift: a device value (the device-resident ESD flow) or the
+// parameter-block value
+__device__ inline double az_shift_of(const ResampleParams& p)
+{
+    return p.az_shift_dev ? *p.az_shift_dev : p.az_shift;
+}
+
+__device__ inline void slave_pos(const ResampleParams& p, int r, int c, int& iy,
+                                 int& fy, int& ix, int& fx)
+{
+#pragma clang fp contract(off)
+    double pn = norm_warp((double)(p.mb[0] + c), (double)p.mb[0], (double)p.mb[1]);
+    double ln = norm_warp((double)(p.mb[2] + r), (double)p.mb[2], (double)p.mb[3]);
+    double ys = polyval6(ln, pn, p.cpm_az) + az_shift_of(p);
+    double xs = polyval6(ln, pn, p.cpm_rg);
+    // keep the int conversion defined for wild polynomials; anything this
+    // far out lands in the zero region anyway
+    ys = fmin(fmax(ys, -1.0e9), 1.0e9);
+    xs = fmin(fmax(xs, -1.0e9), 1.0e9);
+    iy = (int)trunc(ys);
+    fy = (int)((ys - iy) * kInterval + 0.5);
+    ix = (int)trunc(xs);
+    fx = (int)((xs - ix) * kInterval + 0.5);
+}
+
+// 0: outside, 1: edge band, 2: interior (DerampAndResample.cu:383-403)
+__device__ inline int region_of(const ResampleParams& p, int iy, int ix)
+{
+    if (iy > p.sb[3] || iy < p.sb[2] || ix > p.sb[1] || ix < p.sb[0]) return 0;
+    if (iy > p.sb[3] - kHalf - 2 || iy < p.sb[2] + kHalf || ix > p.sb[1] - kHalf - 2 ||
+        ix < p.sb[0] + kHalf)
+        return 1;
+    return 2;
+}
+
+__device__ inline double deramp_phase_direct(const ResampleParams& p, int ya, int xa)
+{
+    double ta = (ya - p.first_line) * p.dt;
+    double d = ta - p.tref[xa];
+    return -kPi * p.kt[xa] * (d * d) - 2.0 * kPi * p.fdc[xa] * ta;
+}
+
+__device__ inline void phase_coeffs(const ResampleParams& p, int xa, double& a,
+                                    double& b, double& c)
+{
+    double kt = p.kt[xa], tr = p.tref[xa], fd = p.fdc[xa];
+    a = -kPi * kt * (tr * tr);
+    b = 2.0 * kPi * p.dt * (kt * tr - fd);
+    c = -kPi * kt * (p.dt * p.dt);
+}
+
+__device__ inline float reduce_2pi(double ph)
+{
+    double k = rint(ph * kInvTwoPi);
+    return (float)fma(-k, kTwoPi, ph);
+}
+
+constexpr int WK_OUT = 51;     // output columns per strip (64-lane wave): 51 + 11 taps + 2 slack
+constexpr int WK_WIN = 16;     // register window rows
+constexpr int WK_KEYS = 4;     // azimuth keys per row served by the shared pass
+constexpr int kWalkWavesPerSimd = 4;   // occupancy the walker is compiled for
+
+__device__ inline void load_taps(const float* lut, int row, float k[kTaps])
+{
+    const float4* q = reinterpret_cast<const float4*>(lut + row * kTaps);
+    float4 a = q[0], b = q[1], c = q[2];
+    k[0] = a.x; k[1] = a.y; k[2] = a.z; k[3] = a.w;
+    k[4] = b.x; k[5] = b.y; k[6] = b.z; k[7] = b.w;
+    k[8] = c.x; k[9] = c.y; k[10] = c.z; k[11] = c.w;
+}
+
+// out = (sr + j si) * exp(-j phi).  The phase goes to revolutions and is
+// reduced to [-1/2, 1/2] in FP64; the hardware sine and cosine take
+// revolutions directly (absolute error about 1.3e-7 on that interval).
+__device__ inline tops_cf32 reramp(float sr, float si, double phi)
+{
+    double t = phi * kInvTwoPi;
+    float tf = (float)(t - rint(t));
+    float sn = __builtin_amdgcn_sinf(tf), cs = __builtin_amdgcn_cosf(tf);
+    tops_cf32 o;
+    o.x = sr * cs + si * sn;
+    o.y = si * cs - sr * sn;
+    return o;
+}
+
+// Interior output computed straight from HBM (outputs the shared pass does
+// not serve).  Inline with rolled loops and one tap loaded per step, so it
+// stays small in registers; it takes plain values, not the parameter block.
+__device__ __forceinline__ tops_cf32 interior_direct(const tops_ci16* slave, const double* kt,
+                                                     const double* tref, const double* fdc,
+                                                     const double* mom, const float* kaz,
+                                                     const float* krg, int s_pixels, int sx0,
+                                                     int sy0, int first_line, double dt, int iy,
+                                                     int fy, int ix, int fx)
+{
+    float sr = 0.f, si = 0.f;
+    double A = 0.0, B = 0.0, C = 0.0;
+#pragma unroll 1
+    for (int j = 0; j < kTaps; j++) {
+        const int ya = iy - kHalf + j;
+        const double y = (double)(ya - first_line);
+        float tr = 0.f, ti = 0.f;
+#pragma unroll 1
+        for (int i = 0; i < kTaps; i++) {
+            const int xa = ix - kHalf + i;
+            const float kpi = krg[fx * kTaps + i];
+            double k = kt[xa], t = tref[xa], f = fdc[xa];
+            double a = -kPi * k * (t * t);
+            double b = 2.0 * kPi * dt * (k * t - f);
+            double c = -kPi * k * (dt * dt);
+            float sn, cs;
+            sincosf(reduce_2pi(fma(y, fma(y, c, b), a)), &sn, &cs);
+            tops_ci16 v = slave[(size_t)(ya - sy0) * s_pixels + (xa - sx0)];
+            float dr = (float)v.re * cs - (float)v.im * sn;
+            float di = (float)v.re * sn + (float)v.im * cs;
+            tr = fmaf(kpi, dr, tr);
+            ti = fmaf(kpi, di, ti);
+            if (j == 0) {
+                A = fma((double)kpi, a, A);
+                B = fma((double)kpi, b, B);
+                C = fma((double)kpi, c, C);
+            }
+        }
+        const float klj = kaz[fy * kTaps + j];
+        sr = fmaf(klj, tr, sr);
+        si = fmaf(klj, ti, si);
+    }
+    double4 m = reinterpret_cast<const double4*>(mom)[fy];
+    double y0 = (double)(iy - first_line);
+    double phi = m.x * A + fma(y0, m.x, m.y) * B + (y0 * y0 * m.x + 2.0 * y0 * m.y + m.z) * C;
+    return reramp(sr, si, phi);
+}
+
+// Column state of one lane: deramp recurrence along its slave column, with
+// the raw samples of the next four lines already requested from HBM.
+struct ColumnWalker {
+    int xcol;            // absolute slave column
+    bool in_box;         // column inside the slave box
+    double pr, pi;       // exp(j phi(ynext))
+    double ir, ii;       // exp(j (b + c (2 ynext + 1)))
+    double qr, qi;       // exp(j 2c)
+    int ynext;           // absolute slave line deramped next
+    int pend[4];         // raw CInt16 of lines ynext .. ynext+3
+
+    // raw CInt16 (packed re | im << 16) of line y of this column, 0 outside
+    // the box (a zero sample deramps to zero)
+    __device__ int fetch(const ResampleParams& p, int y) const
+    {
+        if (!in_box || y < p.sb[2] || y > p.sb[3]) return 0;
+        return reinterpret_cast<const int*>(p.slave)[(size_t)(y - p.sb[2]) * p.s_pixels +
+                                                     (xcol - p.sb[0])];
+    }
+
+    // phi(y) = a + b y + c y^2 (y burst-relative line), anchored at y_abs
+    __device__ void anchor(const ResampleParams& p, int y_abs, double a, double b, double c)
+    {
+        ynext = y_abs;
+        double y = (double)(y_abs - p.first_line);
+        double s, cc;
+        double ph = fma(y, fma(y, c, b), a);
+        sincos(fma(-rint(ph * kInvTwoPi), kTwoPi, ph), &s, &cc);
+        pr = cc; pi = s;
+        double d = fma(c, 2.0 * y + 1.0, b);
+        sincos(fma(-rint(d * kInvTwoPi), kTwoPi, d), &s, &cc);
+        ir = cc; ii = s;
+    }
+
+    // deramped sample of line ynext from its raw value, then advance
+    __device__ float2 deramp(int raw)
+    {
+        const float re = (float)(short)(raw & 0xffff), im = (float)(short)(raw >> 16);
+        const float cr = (float)pr, ci = (float)pi;
+        float2 d = make_float2(re * cr - im * ci, re * ci + im * cr);
+        double nr = pr * ir - pi * ii, ni = pr * ii + pi * ir;
+        pr = nr; pi = ni;
+        double jr = ir * qr - ii * qi, ji = ir * qi + ii * qr;
+        ir = jr; ii = ji;
+        ynext++;
+        return d;
+    }
+
+    // request the four lines after the ones deramped so far
+    __device__ void prefetch(const ResampleParams& p)
+    {
+#pragma unroll
+        for (int k = 0; k < 4; k++) pend[k] = fetch(p, ynext + k);
+    }
+};
+
+// read-only tables addressed with wave-uniform indices
+typedef __attribute__((address_space(4))) const float CFloat;
+typedef __attribute__((address_space(4))) const double CDouble;
+
+// 12-tap azimuth sum of one column over window rows oq .. oq+11: the key's
+// taps pre-shifted by oq into a 16-tap row of the context's table (zeros
+// elsewhere), so one unrolled 16-term sum serves every window offset.  (A
+// 12-term sum per offset behind a scalar branch measured 6% slower on the
+// IW pair.)
+static_assert(kAzShiftTaps == WK_WIN, "a shifted tap row spans the window");
+__device__ __forceinline__ float2 az_taps_shifted(const float2 (&win)[WK_WIN], const float* kaz_sh,
+                                                  int oq, int fyq)
+{
+    CFloat* taps = (CFloat*)(kaz_sh + ((size_t)oq * kLutRows + fyq) * kAzShiftTaps);
+    float2 u = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int t = 0; t < kAzShiftTaps; t++) {
+        const float k = taps[t];
+        u.x = fmaf(k, win[t].x, u.x);
+        u.y = fmaf(k, win[t].y, u.y);
+    }
+    return u;
+}
+
+}  // namespace tops
