@@ -177,12 +177,23 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
         sm.pos[4][lane] = p.cpm_rg[5] * pn * pn;
     }
 
-    // slave column of this lane
-    int iy0, fy0, ix0, fx0;
-    slave_pos(p, r0, c0, iy0, fy0, ix0, fx0);
-    // one column of slack on each side for the slave
-    // column drifting along the strip; same value in every lane
-    const int ucol0 = ix0 - kHalf - 1;
+    // slave column of lane 0.  Output j (lane j) takes its range taps from
+    // lanes ix_j - 5 - ucol0 .. +11, which must lie in 0 .. 63.  With
+    // rel_j = ix_j - j over the strip within [lo, hi] (its corners: the
+    // offset fields are smooth), ucol0 = lo - 5 serves every output when
+    // hi - lo <= 1 (the slave column drifts by at most one column along the
+    // strip's 52 outputs and rows); any other output takes the direct path.
+    int ucol0;
+    {
+        const int cl = min(c0 + WK_OUT - 1, p.m_pixels - 1);
+        const int cc = (lane & 2) ? cl : c0;
+        int iyc, fyc, ixc, fxc;
+        slave_pos(p, (lane & 1) ? r1 - 1 : r0, cc, iyc, fyc, ixc, fxc);
+        const int rel = ixc - (cc - c0);
+        int lo = __builtin_amdgcn_readlane(rel, 0);
+        for (int k = 1; k < 4; k++) lo = min(lo, __builtin_amdgcn_readlane(rel, k));
+        ucol0 = lo - kHalf;
+    }
     ColumnWalker w;
     w.xcol = ucol0 + lane;
     w.in_box = w.xcol >= p.sb[0] && w.xcol <= p.sb[1];

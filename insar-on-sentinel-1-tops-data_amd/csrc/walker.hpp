@@ -68,7 +68,11 @@ __device__ inline float reduce_2pi(double ph)
     return (float)fma(-k, kTwoPi, ph);
 }
 
-constexpr int WK_OUT = 51;     // output columns per strip (64-lane wave): 51 + 11 taps + 2 slack
+// output columns per strip (64-lane wave): 52 + 11 tap columns + 1 column of
+// slack for the slave column drifting along the strip.  52 outputs are 416
+// bytes of CF32: with strips starting on 4-column boundaries and rows of a
+// multiple of 4 samples, every strip row fills whole 32-byte sectors.
+constexpr int WK_OUT = 52;
 constexpr int WK_WIN = 16;     // register window rows
 constexpr int WK_KEYS = 4;     // azimuth keys per row served by the shared pass
 constexpr int kWalkWavesPerSimd = 4;   // occupancy the walker is compiled for
