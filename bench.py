@@ -131,7 +131,9 @@ def bench_geocoreg(tops, torch, pair, L, N, steps, check):
     t = (time.perf_counter() - t0) / steps
     out = {"posts": int(dem.size), "posts_used": int(used.value),
            "mposts_per_s": round(dem.size / t / 1e6, 2), "ms_per_call": round(t * 1e3, 3),
-           "dem": list(dem.shape), "precision": "FP64 orbit, Newton and fit"}
+           "dem": list(dem.shape),
+           "precision": "FP32 DEM projection (relative to an FP64 reference post per 4 x 64 "
+                        "posts), FP64 orbit, zero-Doppler Newton and fit"}
     if check:
         from oracle import oracle as orc
 

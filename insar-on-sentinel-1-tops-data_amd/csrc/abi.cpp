@@ -307,8 +307,7 @@ int tops_geocoreg(const int32_t* d_dem, int lines, int pixels, double lat0, doub
     rc = make_orbit(s, os);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
-    long posts = (long)lines * pixels;
-    int nb = geocoreg_blocks(posts);
+    int nb = geocoreg_blocks(lines, pixels);
     double* d_part = nullptr;
     TOPS_HIP(hipMalloc(&d_part, sizeof(double) * kGeoSums * (size_t)(nb + 1)));
     rc = launch_geocoreg(d_dem, lines, pixels, lat0, lon0, dlat, dlon, nodata, om, os, xmin, xmax,

@@ -1,4 +1,5 @@
-// geocoreg.hip -- DEM-assisted geometric coregistration fit (gfx950, FP64).
+// geocoreg.hip -- DEM-assisted geometric coregistration fit (gfx950; FP32
+// DEM projection, FP64 orbit, Newton solve and fit).
 //
 // Semantics of Geocoreg_cuda.cu: per DEM post (row r, col c) at
 // (lat0 - r*dlat, lon0 + c*dlon) deg with height h (int, nodata skipped):
@@ -12,6 +13,18 @@
 // on [1, L, P, L^2, LP, P^2] of the normalised master position (AMatrixSet,
 // 576-605; cuBLAS Ddgmm/Dgemm + cuSOLVER potrf/potrs, 819-855).
 //
+// Precision split: the DEM projection (llh2xyz) runs in FP32 relative to a
+// reference point per tile of 4 x 64 posts (its centre post at height 0,
+// whose ECEF position Pr each wave forms in FP64).  With dphi = lat - lat_r
+// and dlam = lon - lon_r (at most 2 rows and 32 columns of posts), angle addition
+// gives the small changes of sin / cos from sin(d) and 1 - cos(d), and the
+// prime-vertical radius changes by dN = Nr ((1 - u)^-1/2 - 1) with
+// u = e2 d(sin^2 phi) / (1 - e2 sin^2 lat_r) (series in u).  The post's
+// offset dP from Pr is a sum of FP32 terms of at most a few hundred metres,
+// well under a millimetre in FP32; P = Pr + dP is formed in FP64 for the
+// zero-Doppler Newton solve, which stays FP64 with the orbit polynomials and
+// the normal-equation sums.
+//
 // Here the design matrix is never formed: each thread projects one post
 // against both orbits and adds its weighted contribution to the 21 unique
 // normal-matrix entries, the 12 right-hand-side entries and a count; the
@@ -23,6 +36,7 @@
 namespace tops {
 
 constexpr int GEO_THREADS = 256;
+constexpr int GT_ROWS = 4, GT_COLS = 64;   // posts per tile (one FP64 reference each)
 
 __device__ inline void orbit_pos(const OrbitDev& o, double tn, double s[3])
 {
@@ -76,6 +90,68 @@ __device__ inline void project(const OrbitDev& o, const double P[3], double& lin
     pixel = (sqrt(d0 * d0 + d1 * d1 + d2 * d2) - o.slr_first * kSol) / o.range_spacing;
 }
 
+// sin x and 1 - cos x of a small angle (|x| <= 1e-2 rad; offsets within a
+// tile are a few 1e-4): Taylor terms to x^5 and x^6, so the FP32 results keep
+// their relative precision
+__device__ inline float sin_small(float x)
+{
+    const float x2 = x * x;
+    return x * (1.0f - x2 * (1.0f / 6.0f) * (1.0f - x2 * (1.0f / 20.0f)));
+}
+__device__ inline float omc_small(float x)
+{
+    const float x2 = x * x;
+    return 0.5f * x2 * (1.0f - x2 * (1.0f / 12.0f) * (1.0f - x2 * (1.0f / 30.0f)));
+}
+
+// FP64 constants of llh2xyz at a tile's reference point (lat_r, lon_r, 0)
+// (Geocoreg_cuda.cu:87-99)
+struct TileRef {
+    double pr[3];   // ECEF of the reference post
+    double nv[3];   // the ellipsoid normal there (cos lat cos lon, cos lat sin lon, sin lat)
+    float sr, cr, slr, clr, nr, inv_w2;
+};
+
+__device__ inline TileRef tile_reference(const GeoRef& g, int row, int col)
+{
+    const double d2r = kPi / 180.0;
+    TileRef t;
+    double sr, cr, slr, clr;
+    sincos((g.lat0 - g.dlat * row) * d2r, &sr, &cr);
+    sincos((g.lon0 + g.dlon * col) * d2r, &slr, &clr);
+    const double w2 = 1.0 - g.e2 * sr * sr;
+    const double nr = kEllA / sqrt(w2);
+    t.pr[0] = nr * cr * clr;
+    t.pr[1] = nr * cr * slr;
+    t.pr[2] = nr * g.one_e2 * sr;
+    t.nv[0] = cr * clr;
+    t.nv[1] = cr * slr;
+    t.nv[2] = sr;
+    t.sr = (float)sr; t.cr = (float)cr; t.slr = (float)slr; t.clr = (float)clr;
+    t.nr = (float)nr;
+    t.inv_w2 = (float)(1.0 / w2);
+    return t;
+}
+
+// ECEF of the post at (lat_r + dphi, lon_r + dlam, h) less Pr + h n_r (FP32;
+// the height along the reference normal is added in FP64 by the caller)
+__device__ inline void llh_offset(const TileRef& t, float e2, float one_e2, float dphi,
+                                  float dlam, float h, float d[3])
+{
+    const float sp = sin_small(dphi), sl = sin_small(dlam);
+    const float cp1 = omc_small(dphi), cl1 = omc_small(dlam);   // 1 - cos
+    const float ds = t.cr * sp - t.sr * cp1;       // sin(lat) - sin(lat_r)
+    const float dc = -t.cr * cp1 - t.sr * sp;      // cos(lat) - cos(lat_r)
+    const float dsl = t.clr * sl - t.slr * cl1;    // sin(lon) - sin(lon_r)
+    const float dcl = -t.clr * cl1 - t.slr * sl;   // cos(lon) - cos(lon_r)
+    const float u = e2 * (2.0f * t.sr * ds + ds * ds) * t.inv_w2;
+    const float dn = t.nr * (u * (0.5f + u * (0.375f + u * 0.3125f)));
+    const float c = t.cr + dc, nh = t.nr + h;
+    d[0] = dn * c * (t.clr + dcl) + nh * (t.cr * dcl + dc * t.clr + dc * dcl);
+    d[1] = dn * c * (t.slr + dsl) + nh * (t.cr * dsl + dc * t.slr + dc * dsl);
+    d[2] = dn * one_e2 * (t.sr + ds) + (t.nr * one_e2 + h) * ds;
+}
+
 __device__ inline double wave_sum_d(double v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -83,9 +159,8 @@ __device__ inline double wave_sum_d(double v)
 }
 
 __global__ void __launch_bounds__(GEO_THREADS) geocoreg_kernel(
-    const int32_t* dem, int lines, int pixels, double lat0, double lon0, double dlat,
-    double dlon, int nodata, OrbitDev m, OrbitDev s, int xmin, int xmax, int ymin, int ymax,
-    double* partials, double* lp)
+    const int32_t* dem, int lines, int pixels, GeoRef g, int nodata, OrbitDev m, OrbitDev s,
+    int xmin, int xmax, int ymin, int ymax, double* partials, double* lp)
 {
     __shared__ double red[GEO_THREADS / 64][kGeoSums];
     const long n = (long)lines * pixels;
@@ -93,21 +168,32 @@ __global__ void __launch_bounds__(GEO_THREADS) geocoreg_kernel(
 #pragma unroll
     for (int k = 0; k < kGeoSums; k++) acc[k] = 0.0;
 
-    for (long t = (long)blockIdx.x * GEO_THREADS + threadIdx.x; t < n;
-         t += (long)gridDim.x * GEO_THREADS) {
-        int row = (int)(t / pixels), col = (int)(t - (long)row * pixels);
+    // tiles of GT_ROWS x GT_COLS posts, one post per thread: wave w takes
+    // row w of the tile, lane l its column l
+    static_assert(GT_ROWS * GT_COLS == GEO_THREADS && GT_COLS == 64, "one post per thread");
+    const int tiles_x = (pixels + GT_COLS - 1) / GT_COLS;
+    const long ntiles = (long)tiles_x * ((lines + GT_ROWS - 1) / GT_ROWS);
+    const float e2f = (float)g.e2, one_e2f = (float)g.one_e2;
+    const float dlatf = (float)(g.dlat * kPi / 180.0), dlonf = (float)(g.dlon * kPi / 180.0);
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int tr0 = (int)(tile / tiles_x) * GT_ROWS, tc0 = (int)(tile % tiles_x) * GT_COLS;
+        const int row = tr0 + (threadIdx.x >> 6), col = tc0 + (threadIdx.x & 63);
+        if (row >= lines || col >= pixels) continue;
+        const int rc = tr0 + GT_ROWS / 2, cc = tc0 + GT_COLS / 2;   // reference post
+        const long t = (long)row * pixels + col;
         int hgt = dem[t];
         double ml = -99999999.0, mp = -99999999.0, sl = -99999999.0, sp = -99999999.0;
         if ((float)hgt != (float)nodata) {
-            double lat = (lat0 - dlat * row) * kPi / 180.0;
-            double lon = (lon0 + dlon * col) * kPi / 180.0;
-            double sphi, cphi, slam, clam;
-            sincos(lat, &sphi, &cphi);
-            sincos(lon, &slam, &clam);
-            const double e2 = 1.0 - (kEllB * kEllB / kEllA / kEllA);
-            double h = (double)(float)hgt;
-            double nph = kEllA / sqrt(1.0 - e2 * sphi * sphi) + h;
-            double P[3] = {nph * cphi * clam, nph * cphi * slam, (nph - e2 * (nph - h)) * sphi};
+            const TileRef ref = tile_reference(g, rc, cc);
+            // angles from the reference post: a few posts times the spacing
+            const float dphi = -(float)(row - rc) * dlatf;
+            const float dlam = (float)(col - cc) * dlonf;
+            float d[3];
+            llh_offset(ref, e2f, one_e2f, dphi, dlam, (float)hgt, d);
+            const double hd = (double)(float)hgt;
+            double P[3] = {fma(hd, ref.nv[0], ref.pr[0]) + (double)d[0],
+                           fma(hd, ref.nv[1], ref.pr[1]) + (double)d[1],
+                           fma(hd, ref.nv[2], ref.pr[2]) + (double)d[2]};
             project(m, P, ml, mp);
             project(s, P, sl, sp);
             if (ml >= ymin && ml <= ymax && mp >= xmin && mp <= xmax) {
@@ -165,9 +251,11 @@ __global__ void __launch_bounds__(256) geocoreg_final_kernel(double* partials, i
     if (t == 0) partials[(size_t)nblocks * kGeoSums + k] = sh[0];
 }
 
-int geocoreg_blocks(long posts)
+// one workgroup per tile of GT_ROWS x GT_COLS posts, at most 2048 (the
+// grid-stride loop takes the rest)
+int geocoreg_blocks(int lines, int pixels)
 {
-    long b = (posts + GEO_THREADS - 1) / GEO_THREADS;
+    long b = (long)((lines + GT_ROWS - 1) / GT_ROWS) * ((pixels + GT_COLS - 1) / GT_COLS);
     if (b > 2048) b = 2048;
     if (b < 1) b = 1;
     return (int)b;
@@ -178,9 +266,15 @@ int launch_geocoreg(const int32_t* dem, int lines, int pixels, double lat0, doub
                     const OrbitDev& s, int xmin, int xmax, int ymin, int ymax,
                     double* d_partials, int nblocks, double* d_lp, hipStream_t st)
 {
+    GeoRef g;
+    g.lat0 = lat0;
+    g.lon0 = lon0;
+    g.dlat = dlat;
+    g.dlon = dlon;
+    g.e2 = 1.0 - (kEllB * kEllB / kEllA / kEllA);   // b_ell_e2, cuConstants.cuh:13
+    g.one_e2 = 1.0 - g.e2;
     hipLaunchKernelGGL(geocoreg_kernel, dim3(nblocks), dim3(GEO_THREADS), 0, st, dem, lines,
-                       pixels, lat0, lon0, dlat, dlon, nodata, m, s, xmin, xmax, ymin, ymax,
-                       d_partials, d_lp);
+                       pixels, g, nodata, m, s, xmin, xmax, ymin, ymax, d_partials, d_lp);
     TOPS_HIP(hipGetLastError());
     hipLaunchKernelGGL(geocoreg_final_kernel, dim3(kGeoSums), dim3(256), 0, st, d_partials,
                        nblocks);

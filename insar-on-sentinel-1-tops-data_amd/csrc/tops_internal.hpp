@@ -177,12 +177,18 @@ struct OrbitDev {
     double mid_time, dt, first_line_time, slr_first, range_spacing, az_init;
     int line_offset;
 };
+// DEM grid and ellipsoid of the geocoreg kernel (the FP32 projection's
+// reference points are formed per tile from these)
+struct GeoRef {
+    double lat0, lon0, dlat, dlon;   // degrees: post (r, c) at (lat0 - r dlat, lon0 + c dlon)
+    double e2, one_e2;               // e^2 and 1 - e^2 of WGS84
+};
 int launch_geocoreg(const int32_t* dem, int lines, int pixels, double lat0,
                     double lon0, double dlat, double dlon, int nodata,
                     const OrbitDev& m, const OrbitDev& s, int xmin, int xmax,
                     int ymin, int ymax, double* d_partials, int nblocks,
                     double* d_lp, hipStream_t st);
-int geocoreg_blocks(long posts);
+int geocoreg_blocks(int lines, int pixels);
 constexpr int kGeoSums = 34;   // 21 N + 6 rhs_az + 6 rhs_rg + count
 
 }  // namespace tops
