@@ -213,6 +213,25 @@ def read_sq(kernel, workload):
         return None
 
 
+def tuner_record(L, plan, chosen):
+    """The candidates tops_plan_tune timed ([strip_rows, coherence rows per
+    block, ms per run]; strip_rows 0 = per-burst launches, > 0 = batched
+    strips of that height, < 0 = the fused launch over row bands), the
+    choice, and the untuned default (per-burst launches, 32 coherence rows)."""
+    timed = []
+    for i in range(L.tops_plan_tune_count(plan)):
+        r, cr, ms = C.c_int(), C.c_int(), C.c_float()
+        if L.tops_plan_tune_entry(plan, i, C.byref(r), C.byref(cr), C.byref(ms)) == 0:
+            timed.append([r.value, cr.value, round(ms.value, 4)])
+    pick = [chosen, L.tops_plan_coh_rows(plan)]
+    ms_of = {(r, c): m for r, c, m in timed}
+    default = ms_of.get((0, 32))
+    tuned = ms_of.get(tuple(pick))
+    return {"timed": timed, "chosen": pick, "chosen_ms": tuned, "default": [0, 32],
+            "default_ms": default,
+            "speedup_vs_default": round(default / tuned, 3) if default and tuned else None}
+
+
 def run_batched(args, tops, torch):
     """A batched workload through one tuned plan replayed as a hipGraph:
       iw_pair: 27 bursts (3 subswaths x 9; 21000 / 25000 / 24500 samples,
@@ -288,6 +307,9 @@ def run_batched(args, tops, torch):
         raise RuntimeError(N.last_error())
     best = C.c_int(0)
     N.check(L.tops_plan_tune(plan, sp, C.byref(best)), "tune")
+    tuner = tuner_record(L, plan, best.value)
+    print(f"[bench] {workload}: tuner " + ", ".join(f"{r}/{c}: {m}" for r, c, m in tuner["timed"]),
+          file=sys.stderr, flush=True)
     names = [L.tops_plan_kernel_name(plan, k).decode() for k in range(3)]
     N.check(L.tops_plan_capture_graph(plan, sp), "graph")
     print(f"[bench] {workload}: schedule {best.value} ({', '.join(n for n in names if n)})",
@@ -375,6 +397,7 @@ def run_batched(args, tops, torch):
                    "widths": sorted({b.width for b in bursts}),
                    "lines": [min(b.lines for b in bursts), max(b.lines for b in bursts)],
                    "taps": 12, "coh_window": [3, 10], "schedule_strip_rows": best.value,
+                   "coh_rows": L.tops_plan_coh_rows(plan), "tuner": tuner,
                    "kernels": [n for n in names if n], "hip_graph": True,
                    "admitted_batch": admitted, "hbm_free_gb": round(free / 1e9, 1),
                    "precision": "FP32 sinc MACs and window sums, FP64 phase reduced mod 2pi",

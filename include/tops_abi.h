@@ -219,22 +219,35 @@ typedef struct {
 typedef struct tops_plan tops_plan;
 /* A plan resamples every burst and forms its coherence (window coh_win_az
  * x coh_win_rg starting (win-1)/2 before the sample) on one stream, on one
- * of two schedules: strip_rows <= 0, one resample and one coherence launch
- * per burst with heights sized to each burst; strip_rows > 0, one batched
- * resample launch over every burst's strips of that height, then one
- * batched coherence launch. */
+ * of three schedules: strip_rows == 0, one resample and one coherence
+ * launch per burst with heights sized to each burst; strip_rows > 0, one
+ * batched resample launch over every burst's strips of that height, then
+ * one batched coherence launch; strip_rows < 0, one persistent launch that
+ * resamples and forms the coherence of row bands of -strip_rows lines (the
+ * 3 az x 10 rg window; other windows take the batched schedule at height
+ * -strip_rows).  No counterpart in the reference, which runs one burst per
+ * call. */
 tops_plan* tops_plan_create(tops_ctx* ctx, const tops_burst_item* items, int n_items,
                             int coh_win_az, int coh_win_rg, int strip_rows);
 int tops_plan_run(tops_plan* plan, void* stream);
 /* Capture the plan's launches in a hipGraph on a non-default stream; later
  * tops_plan_run calls replay it. */
 int tops_plan_capture_graph(tops_plan* plan, void* stream);
-/* Time the candidate schedules on this plan (per-burst, and batched at
- * several strip heights) and keep the fastest; *best_strip_rows gets its
- * strip_rows value (0 = per-burst).  Cached in the file named by
+/* Time the candidate schedules on this plan (per-burst, batched at several
+ * strip heights, fused at several row bands) and keep the fastest; then,
+ * for a batched schedule with the 3 az x 10 rg window, time the batched
+ * coherence's rows per block and keep the fastest.  *best_strip_rows gets
+ * the chosen strip_rows value.  Cached in the file named by
  * TOPS_TUNE_CACHE when set. */
 int tops_plan_tune(tops_plan* plan, void* stream, int* best_strip_rows);
 int tops_plan_strip_rows(const tops_plan* plan);
+/* Rows per block of the plan's batched coherence walk (default 32). */
+int tops_plan_coh_rows(const tops_plan* plan);
+/* The candidates the last tops_plan_tune timed, in order: entry i gives
+ * its strip_rows, coherence rows per block and mean ms per run. */
+int tops_plan_tune_count(const tops_plan* plan);
+int tops_plan_tune_entry(const tops_plan* plan, int i, int* strip_rows, int* coh_rows,
+                         float* ms);
 /* Durations of the plan's launches: `reps` eager runs of its schedule on
  * `stream` with an event pair around every launch.  ms[k] (k = 0 resample,
  * 1 coherence, 2 fused resample + coherence; 3 entries) gets the mean per

@@ -98,8 +98,10 @@ __global__ void __launch_bounds__(256) coherence_batch_kernel(const CohItem* ite
 }
 
 // Walking form for the 3 az x 10 rg window centred as the stage layer
-// centres it (dx = 4, dy = 1): 256 threads = 4 waves; a wave covers
-// 256 columns x CW_ROWS rows, the workgroup 256 columns x 4 CW_ROWS rows.
+// centres it (dx = 4, dy = 1): 256 threads = 4 waves, 2 columns per
+// thread, so a workgroup walks down a band of CW_BAND = 512 columns over
+// the rows it is given (CW_ROWS by default in the batched form, a plan may
+// tune it).
 using CohWalk3x10 = CohWalk<10, 3, 4, 1>;
 
 // Workgroup (bx, by): band bx (CW_BAND columns), output rows by*rows ..
@@ -127,13 +129,14 @@ coherence_walk_kernel(const tops_ci16* M, const tops_cf32* S, int w, int h, floa
     coherence_walk_block(M, S, w, h, coh, bx, by, vec, rows, sm);
 }
 
-// Batched form: work[i] = {item, band, row block, vec}, CW_ROWS rows per block.
+// Batched form: work[i] = {item, band, row block, vec}, `rows` rows per block.
 // The table (bands fastest within a row block) is dealt to the XCDs in
 // contiguous ranges, so neighbouring blocks, whose halo rows and columns
 // overlap, share an L2; table and items are read through the constant
 // address space (scalar loads: read-only, wave-uniform indices).
 __global__ void __launch_bounds__(CW_THREADS) coherence_walk_batch_kernel(const CohItem* items,
-                                                                          const int4* work)
+                                                                          const int4* work,
+                                                                          int rows)
 {
     __shared__ CohWalk3x10::Lds sm;
     typedef __attribute__((address_space(4))) const int CInt;
@@ -141,7 +144,7 @@ __global__ void __launch_bounds__(CW_THREADS) coherence_walk_batch_kernel(const 
     const CInt* wk = (const CInt*)work + 4 * xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int item = wk[0], band = wk[1], block = wk[2], vec = wk[3];
     CCohItem* it = (CCohItem*)items + item;
-    coherence_walk_block(it->m, it->s, it->w, it->h, it->coh, band, block, vec, CW_ROWS, sm);
+    coherence_walk_block(it->m, it->s, it->w, it->h, it->coh, band, block, vec, rows, sm);
 }
 
 bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az)
@@ -164,10 +167,10 @@ int coherence_walk_vec(const void* m, const void* s, const void* coh, int w)
            ((uintptr_t)coh % 8 == 0);
 }
 
-int coherence_walk_count(int w, int h, int* nx, int* ny)
+int coherence_walk_count(int w, int h, int rows, int* nx, int* ny)
 {
     *nx = (w + CW_BAND - 1) / CW_BAND;
-    *ny = (h + CW_ROWS - 1) / CW_ROWS;
+    *ny = (h + rows - 1) / rows;
     return *nx * *ny;
 }
 
@@ -198,12 +201,14 @@ int coherence_tile_count(int w, int h, int* nx, int* ny)
 }
 
 int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_work, int dx, int dy,
-                           int win_rg, int win_az, hipStream_t st)
+                           int win_rg, int win_az, int walk_rows, hipStream_t st)
 {
     if (n_work <= 0) return TOPS_OK;
     if (coherence_walk_fits(dx, dy, win_rg, win_az)) {
+        if (walk_rows <= 0)
+            return set_error(TOPS_ERR_ARG, "batched coherence: %d rows per block", walk_rows);
         hipLaunchKernelGGL(coherence_walk_batch_kernel, dim3(n_work), dim3(CW_THREADS), 0, st,
-                           d_items, d_work);
+                           d_items, d_work, walk_rows);
         TOPS_HIP(hipGetLastError());
         return TOPS_OK;
     }

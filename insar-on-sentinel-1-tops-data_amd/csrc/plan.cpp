@@ -2,8 +2,10 @@
 // of an IW pair, or a stack of slaves against one master): one strip table
 // covers every burst, so the whole deramp -> resample -> reramp pass is one
 // launch and the coherence pass another; the pair of launches can be
-// captured once in a hipGraph and replayed.  The strip height is a tuning
-// parameter chosen by timing the candidates on the plan's own bursts.
+// captured once in a hipGraph and replayed.  A third schedule fuses both
+// passes in one persistent launch.  The schedule, its strip height or row
+// band, and the coherence walk's rows per block are tuning parameters chosen
+// by timing the candidates on the plan's own bursts.
 #include <algorithm>
 #include <stdio.h>
 #include <stdlib.h>
@@ -14,6 +16,8 @@
 #include "tops_internal.hpp"
 
 using namespace tops;
+
+constexpr int kDefaultCohRows = 32;   // the coherence walk's rows per block before tuning
 
 struct tops_plan {
     tops_ctx* ctx = nullptr;
@@ -37,6 +41,12 @@ struct tops_plan {
     // the batched coherence launch serves the window; otherwise the batched
     // schedule forms coherence with one launch per burst
     bool coh_batched = true;
+    bool coh_walk = false;     // the window is the walking form's (3 az x 10 rg)
+    int coh_rows = kDefaultCohRows;   // output rows per block of the batched coherence walk
+    // the tuner's record: every candidate it timed (strip_rows, coh_rows, ms
+    // per run of the plan)
+    struct TuneEntry { int strip_rows, coh_rows; float ms; };
+    std::vector<TuneEntry> tune_log;
     // fused schedule: coherence item of each burst (-1: none), the work
     // table and the queue counters of the persistent launch
     bool fused_ok = false;
@@ -50,6 +60,8 @@ struct tops_plan {
 };
 
 namespace {
+
+int fill_coh_work(tops_plan* p);
 
 int fill_params(tops_plan* p)
 {
@@ -86,10 +98,11 @@ int fill_params(tops_plan* p)
     TOPS_HIP(hipMemcpy(p->d_params, prm.data(), sizeof(ResampleParams) * prm.size(),
                        hipMemcpyHostToDevice));
     std::vector<CohItem> ci;
-    std::vector<int4> wc;
     p->coh_of.assign(p->items.size(), -1);
     p->coh_batched = coherence_batch_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2,
                                           p->coh_win_rg, p->coh_win_az);
+    p->coh_walk = coherence_walk_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2,
+                                      p->coh_win_rg, p->coh_win_az);
     for (size_t i = 0; i < p->items.size(); i++) {
         const tops_burst_item& it = p->items[i];
         if (!it.master || !it.coh) continue;
@@ -100,17 +113,6 @@ int fill_params(tops_plan* p)
         c.coh = it.coh;
         c.w = prm[i].m_pixels;
         c.h = prm[i].m_lines;
-        int nx, ny, tag = 0;
-        if (coherence_walk_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2, p->coh_win_rg,
-                                p->coh_win_az)) {
-            coherence_walk_count(c.w, c.h, &nx, &ny);
-            tag = coherence_walk_vec(c.m, c.s, c.coh, c.w);
-        } else {
-            coherence_tile_count(c.w, c.h, &nx, &ny);
-        }
-        if (p->coh_batched)
-            for (int y = 0; y < ny; y++)
-                for (int x = 0; x < nx; x++) wc.push_back(make_int4((int)ci.size(), x, y, tag));
         ci.push_back(c);
     }
     p->h_coh = ci;
@@ -129,10 +131,37 @@ int fill_params(tops_plan* p)
         TOPS_HIP(hipMalloc(&p->d_queue, 2 * sizeof(unsigned)));
         TOPS_HIP(hipMemset(p->d_queue, 0, 2 * sizeof(unsigned)));
     }
-    p->n_work_coh = (int)wc.size();
-    if (!wc.empty()) {
+    if (p->coh_batched && !ci.empty()) {
         TOPS_HIP(hipMalloc(&p->d_coh, sizeof(CohItem) * ci.size()));
         TOPS_HIP(hipMemcpy(p->d_coh, ci.data(), sizeof(CohItem) * ci.size(), hipMemcpyHostToDevice));
+    }
+    return fill_coh_work(p);
+}
+
+// work table of the batched coherence launch: {coherence item, column band,
+// row block, vec}, blocks of coh_rows rows for the walking form, 16 x 64
+// tiles otherwise
+int fill_coh_work(tops_plan* p)
+{
+    if (p->d_work_coh) (void)hipFree(p->d_work_coh);
+    p->d_work_coh = nullptr;
+    p->n_work_coh = 0;
+    if (!p->coh_batched) return TOPS_OK;
+    std::vector<int4> wc;
+    for (size_t k = 0; k < p->h_coh.size(); k++) {
+        const CohItem& c = p->h_coh[k];
+        int nx, ny, tag = 0;
+        if (p->coh_walk) {
+            coherence_walk_count(c.w, c.h, p->coh_rows, &nx, &ny);
+            tag = coherence_walk_vec(c.m, c.s, c.coh, c.w);
+        } else {
+            coherence_tile_count(c.w, c.h, &nx, &ny);
+        }
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) wc.push_back(make_int4((int)k, x, y, tag));
+    }
+    p->n_work_coh = (int)wc.size();
+    if (!wc.empty()) {
         TOPS_HIP(hipMalloc(&p->d_work_coh, sizeof(int4) * wc.size()));
         TOPS_HIP(hipMemcpy(p->d_work_coh, wc.data(), sizeof(int4) * wc.size(), hipMemcpyHostToDevice));
     }
@@ -266,7 +295,7 @@ int enqueue(tops_plan* p, hipStream_t s, Marks* mk = nullptr)
     if (p->coh_batched)
         return marked(mk, s, kKindCoherence, [&] {
             return launch_coherence_batch(p->d_coh, p->d_work_coh, p->n_work_coh, dx, dy,
-                                          p->coh_win_rg, p->coh_win_az, s);
+                                          p->coh_win_rg, p->coh_win_az, p->coh_rows, s);
         });
     for (const CohItem& c : p->h_coh)
         if ((rc = coh_one(c))) return rc;
@@ -419,13 +448,47 @@ const char* tops_plan_kernel_name(const tops_plan* p, int kind)
     return walk ? "coherence_walk_batch_kernel" : "coherence_batch_kernel";
 }
 
-// Autotuner: times every candidate schedule (per-burst launches, or one
-// batched launch at each candidate strip height) on the plan's bursts,
-// resample and coherence together (warm-up + 3 timed runs each, hipEvents
-// on `stream`), and keeps the fastest.  With TOPS_TUNE_CACHE set, the
-// choice is read from / appended to that text file ("<key> <schedule>"
-// lines; the key names the burst count and coherence window and hashes the
-// burst shapes).
+}  // extern "C"
+
+// Autotuner, in two stages on the plan's own bursts (warm-up + 3 timed runs
+// per candidate, hipEvents on `stream`, resample and coherence together):
+//   1. the schedule: per-burst launches, one batched launch at each
+//      candidate strip height, or the fused launch at each candidate row
+//      band (3 az x 10 rg window);
+//   2. for a batched schedule with the walking coherence (3 az x 10 rg), the
+//      coherence walk's rows per block.
+// It keeps the fastest of each stage and records every candidate timed
+// (tops_plan_tune_entry).  With TOPS_TUNE_CACHE set, the choice is read from
+// / appended to that text file ("<key> <strip_rows> <coh_rows>" lines; the
+// key names the burst count and coherence window and hashes the burst
+// shapes).
+namespace {
+
+// mean ms per run of the plan's current schedule over 3 runs after a warm-up
+int time_runs(tops_plan* p, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* out)
+{
+    int rc = enqueue(p, s);   // warm-up
+    if (rc) return rc;
+    float tot = 0.f;
+    for (int k = 0; k < 3; k++) {
+        hipError_t e = hipEventRecord(e0, s);
+        if (e == hipSuccess) rc = enqueue(p, s);
+        if (rc) return rc;
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) return hip_fail(e, "tops_plan_tune timing", __FILE__, __LINE__);
+        tot += ms;
+    }
+    *out = tot / 3.f;
+    return TOPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
 {
     if (!p) return set_error(TOPS_ERR_ARG, "tops_plan_tune: null plan");
@@ -447,20 +510,25 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
     snprintf(kb, sizeof(kb), "plan_n%zu_w%dx%d_%016llx", p->items.size(), p->coh_win_az,
              p->coh_win_rg, (unsigned long long)hsh);
     const std::string key = kb;
+    p->tune_log.clear();
     const char* cache = getenv("TOPS_TUNE_CACHE");
     if (cache) {
         FILE* f = fopen(cache, "r");
         if (f) {
-            char k[512];
-            int v;
-            while (fscanf(f, "%511s %d", k, &v) == 2)
-                if (key == k && (v >= 0 || p->fused_ok)) {
+            char line[640], k[512];
+            int v, cr;
+            while (fgets(line, sizeof(line), f)) {
+                if (sscanf(line, "%511s %d %d", k, &v, &cr) != 3) continue;
+                if (key == k && (v >= 0 || p->fused_ok) && cr > 0) {
                     fclose(f);
                     drop_graph(p);
                     p->strip_rows = v;
+                    p->coh_rows = cr;
                     if (best_rows) *best_rows = v;
-                    return fill_work(p);
+                    int rc = fill_work(p);
+                    return rc ? rc : fill_coh_work(p);
                 }
+            }
             fclose(f);
         }
     }
@@ -481,6 +549,7 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
             if (std::find(cand.begin(), cand.end(), h) == cand.end()) cand.push_back(h);
         }
     }
+    p->coh_rows = kDefaultCohRows;
     float best = 1e30f;
     int best_r = p->strip_rows;
     hipEvent_t e0, e1;
@@ -490,38 +559,57 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
     int rc = TOPS_OK;
     for (int c : cand) {
         p->strip_rows = c;
-        if ((rc = fill_work(p))) break;
-        if ((rc = enqueue(p, s))) break;   // warm-up
-        float tot = 0.f;
-        for (int k = 0; k < 3 && !rc; k++) {
-            hipError_t e = hipEventRecord(e0, s);
-            if (e == hipSuccess) rc = enqueue(p, s);
-            if (e == hipSuccess) e = hipEventRecord(e1, s);
-            if (e == hipSuccess) e = hipEventSynchronize(e1);
-            float ms = 0.f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-            if (e != hipSuccess) rc = hip_fail(e, "tops_plan_tune timing", __FILE__, __LINE__);
-            tot += ms;
-        }
-        if (rc) break;
-        if (tot < best) {
-            best = tot;
+        float ms = 0.f;
+        if ((rc = fill_work(p)) || (rc = time_runs(p, s, e0, e1, &ms))) break;
+        p->tune_log.push_back({c, p->coh_rows, ms});
+        if (ms < best) {
+            best = ms;
             best_r = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     p->strip_rows = best_r;
     if (!rc) rc = fill_work(p);
+    // stage 2: the batched coherence walk's rows per block
+    if (!rc && best_r > 0 && p->coh_batched && p->coh_walk) {
+        int best_c = p->coh_rows;
+        for (int cr : {16, 64, 128, 256}) {
+            p->coh_rows = cr;
+            float ms = 0.f;
+            if ((rc = fill_coh_work(p)) || (rc = time_runs(p, s, e0, e1, &ms))) break;
+            p->tune_log.push_back({best_r, cr, ms});
+            if (ms < best) {
+                best = ms;
+                best_c = cr;
+            }
+        }
+        p->coh_rows = best_c;
+        if (!rc) rc = fill_coh_work(p);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (!rc && cache) {
         FILE* f = fopen(cache, "a");
         if (f) {
-            fprintf(f, "%s %d\n", key.c_str(), best_r);
+            fprintf(f, "%s %d %d\n", key.c_str(), p->strip_rows, p->coh_rows);
             fclose(f);
         }
     }
-    if (best_rows) *best_rows = best_r;
+    if (best_rows) *best_rows = p->strip_rows;
     return rc;
+}
+
+int tops_plan_coh_rows(const tops_plan* p) { return p ? p->coh_rows : 0; }
+
+int tops_plan_tune_count(const tops_plan* p) { return p ? (int)p->tune_log.size() : 0; }
+
+int tops_plan_tune_entry(const tops_plan* p, int i, int* strip_rows, int* coh_rows, float* ms)
+{
+    if (!p || i < 0 || i >= (int)p->tune_log.size())
+        return set_error(TOPS_ERR_ARG, "tops_plan_tune_entry: no entry %d", i);
+    if (strip_rows) *strip_rows = p->tune_log[i].strip_rows;
+    if (coh_rows) *coh_rows = p->tune_log[i].coh_rows;
+    if (ms) *ms = p->tune_log[i].ms;
+    return TOPS_OK;
 }
 
 }  // extern "C"

@@ -240,14 +240,21 @@ __device__ inline void tile_window_sums_shared(const tops_ci16* M, const tops_cf
             center[k] = make_float2(v.x, v.y);
         }
     }
-    // range sums of row rr = task % PR, windows 8g .. 8g+7 (g = task / PR):
-    // the lanes of a wave take different rows, whose starts (PC float4
-    // apart; PC is odd for even windows such as the reference's 24) fall in
-    // different LDS banks, and the range sums are stored TW_HSTRIDE (odd)
-    // float4 apart per row for the same reason
+    // range sums of row rr = task % PRP, windows 8g .. 8g+7 (g = task / PRP),
+    // with the rows padded to PRP, a multiple of 8: the lanes of a wave take
+    // different rows, whose starts (PC float4 apart; PC is odd for even
+    // windows such as the reference's 24) fall in different LDS banks, and the
+    // range sums are stored TW_HSTRIDE (odd) float4 apart per row for the same
+    // reason.  With PRP a multiple of 8, a 16-lane LDS phase that spans two
+    // runs takes the same 8 row offsets of both, shifted by the 8-float4 run
+    // width, so its 16 float4 accesses still fall in 16 different bank
+    // groups.  (Unpadded, the 21 rows of the 6 x 24 window put two rows of a
+    // phase in one bank group: 16.95 M conflict cycles per 8 x 100 x 21000
+    // call, 1.18 M padded.)
+    const int PRP = (PR + 7) & ~7;
     const int task = threadIdx.x;
-    const bool has = task < PR * 8;
-    const int trow = task % PR, trun = task / PR;
+    const int trow = task % PRP, trun = task / PRP;
+    const bool has = trow < PR && trun < 8;
     float4 hs[8];
     if (has) {
         const float4* q = smem + trow * PC + trun * 8;

@@ -135,14 +135,15 @@ struct CohItem {
     int w, h;
 };
 int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_work, int dx,
-                           int dy, int win_rg, int win_az, hipStream_t st);
+                           int dy, int win_rg, int win_az, int walk_rows, hipStream_t st);
 int coherence_tile_count(int w, int h, int* nx, int* ny);
 // walking form (3 az x 10 rg window, dx = 4, dy = 1); batched work items are
-// {item, column band, row block, coherence_walk_vec(...)}
+// {item, column band, row block, coherence_walk_vec(...)}, blocks of
+// walk_rows output rows
 bool coherence_walk_fits(int dx, int dy, int win_rg, int win_az);
 bool coherence_batch_fits(int dx, int dy, int win_rg, int win_az);
 int coherence_walk_vec(const void* m, const void* s, const void* coh, int w);
-int coherence_walk_count(int w, int h, int* nx, int* ny);
+int coherence_walk_count(int w, int h, int rows, int* nx, int* ny);
 
 // launchers (fused.hip): one persistent launch of deramp -> resample ->
 // reramp -> coherence over work items {burst, column strip, first row, rows}

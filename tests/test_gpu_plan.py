@@ -126,6 +126,21 @@ def test_plan_tune_and_graph_replay(tops, dev, batch):
     # (negative)
     assert best.value == 0 or 16 <= abs(best.value) <= 4096
     assert L.tops_plan_strip_rows(plan) == best.value
+    # the tuner's record: every candidate timed; the choice is the fastest
+    n = L.tops_plan_tune_count(plan)
+    assert n >= 10
+    log = []
+    for i in range(n):
+        r, cr, ms = C.c_int(), C.c_int(), C.c_float()
+        N.check(L.tops_plan_tune_entry(plan, i, C.byref(r), C.byref(cr), C.byref(ms)))
+        log.append((r.value, cr.value, ms.value))
+    assert all(ms > 0 for _, _, ms in log)
+    chosen = (best.value, L.tops_plan_coh_rows(plan))
+    assert L.tops_plan_coh_rows(plan) in (16, 32, 64, 128, 256)
+    assert min(log, key=lambda e: e[2])[:2] == chosen, (log, chosen)
+    assert L.tops_plan_tune_entry(plan, n, None, None, None) != 0
+    L.tops_clear_error()
+    print("  plan: tuner " + ", ".join(f"{r}/{cr}: {ms:.3f}" for r, cr, ms in log), flush=True)
     N.check(L.tops_plan_run(plan, sp))
     s.synchronize()
     eager = [(dev.get(d["out"], np.complex64, (b["L"], b["P"])).copy(),
