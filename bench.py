@@ -214,19 +214,20 @@ def read_sq(kernel, workload):
 
 def tuner_record(L, plan, chosen):
     """The candidates tops_plan_tune timed ([strip_rows, coherence rows per
-    block, ms per run]; strip_rows 0 = per-burst launches, > 0 = batched
-    strips of that height, < 0 = the fused launch over row bands), the
-    choice, and the untuned default (per-burst launches, 32 coherence rows)."""
+    block, burst groups, ms per run]; strip_rows 0 = per-burst launches,
+    > 0 = batched strips of that height, < 0 = the fused launch over row
+    bands), the choice, and the untuned default (per-burst launches, 32
+    coherence rows, one group)."""
     timed = []
     for i in range(L.tops_plan_tune_count(plan)):
-        r, cr, ms = C.c_int(), C.c_int(), C.c_float()
-        if L.tops_plan_tune_entry(plan, i, C.byref(r), C.byref(cr), C.byref(ms)) == 0:
-            timed.append([r.value, cr.value, round(ms.value, 4)])
-    pick = [chosen, L.tops_plan_coh_rows(plan)]
-    ms_of = {(r, c): m for r, c, m in timed}
-    default = ms_of.get((0, 32))
+        r, cr, g, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        if L.tops_plan_tune_entry(plan, i, C.byref(r), C.byref(cr), C.byref(g), C.byref(ms)) == 0:
+            timed.append([r.value, cr.value, g.value, round(ms.value, 4)])
+    pick = [chosen, L.tops_plan_coh_rows(plan), L.tops_plan_groups(plan)]
+    ms_of = {(r, c, g): m for r, c, g, m in timed}
+    default = ms_of.get((0, 32, 1))
     tuned = ms_of.get(tuple(pick))
-    return {"timed": timed, "chosen": pick, "chosen_ms": tuned, "default": [0, 32],
+    return {"timed": timed, "chosen": pick, "chosen_ms": tuned, "default": [0, 32, 1],
             "default_ms": default,
             "speedup_vs_default": round(default / tuned, 3) if default and tuned else None}
 
@@ -332,7 +333,9 @@ def run_batched(args, tops, torch):
     best = C.c_int(0)
     N.check(L.tops_plan_tune(plan, sp, C.byref(best)), "tune")
     tuner = tuner_record(L, plan, best.value)
-    print(f"[bench] {workload}: tuner " + ", ".join(f"{r}/{c}: {m}" for r, c, m in tuner["timed"]),
+    if args.groups > 0:
+        N.check(L.tops_plan_set_groups(plan, args.groups), "groups")
+    print(f"[bench] {workload}: tuner " + ", ".join(f"{r}/{c}/{g}: {m}" for r, c, g, m in tuner["timed"]),
           file=sys.stderr, flush=True)
     names = [L.tops_plan_kernel_name(plan, k).decode() for k in range(3)]
     N.check(L.tops_plan_capture_graph(plan, sp), "graph")
@@ -346,7 +349,13 @@ def run_batched(args, tops, torch):
         N.check(L.tops_plan_run(plan, sp))
     torch.cuda.synchronize()
     step_s = (time.perf_counter() - t0) / args.steps
-    # the kernels of the schedule, with HIP events on the plan's stream
+    # the kernels of the schedule, with HIP events on the plan's stream.  With
+    # burst groups a group's coherence runs beside the next group's
+    # resampling, which stretches both launches: the launch durations (and
+    # the roofline) are taken with one group, each kernel on its own.
+    groups = L.tops_plan_groups(plan)
+    if groups > 1:
+        N.check(L.tops_plan_set_groups(plan, 1), "groups")
     ms = (C.c_float * 3)()
     nl = (C.c_int * 3)()
     N.check(L.tops_plan_time_launches(plan, sp, max(3, min(args.steps, 10)), ms, nl), "timing")
@@ -406,7 +415,8 @@ def run_batched(args, tops, torch):
                    "widths": sorted({b.width for b in bursts}),
                    "lines": [min(b.lines for b in bursts), max(b.lines for b in bursts)],
                    "taps": 12, "coh_window": [3, 10], "schedule_strip_rows": best.value,
-                   "coh_rows": L.tops_plan_coh_rows(plan), "tuner": tuner,
+                   "coh_rows": L.tops_plan_coh_rows(plan), "burst_groups": groups,
+                   "tuner": tuner,
                    "kernels": [n for n in names if n], "hip_graph": True,
                    "admitted_batch": admitted, "hbm_free_gb": round(free / 1e9, 1),
                    "precision": "FP32 sinc MACs and window sums, FP64 phase reduced mod 2pi",
@@ -610,6 +620,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-side-stages", action="store_true",
                     help="iw_pair: skip the geocoreg and ESD figures")
+    ap.add_argument("--groups", type=int, default=0,
+                    help="iw_pair / stack: burst groups of the plan (0: the tuner's choice)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (twice) and exit: "
                          "the reference kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE")

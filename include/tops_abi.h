@@ -235,19 +235,28 @@ int tops_plan_run(tops_plan* plan, void* stream);
 int tops_plan_capture_graph(tops_plan* plan, void* stream);
 /* Time the candidate schedules on this plan (per-burst, batched at several
  * strip heights, fused at several row bands) and keep the fastest; then,
- * for a batched schedule with the 3 az x 10 rg window, time the batched
- * coherence's rows per block and keep the fastest.  *best_strip_rows gets
- * the chosen strip_rows value.  Cached in the file named by
- * TOPS_TUNE_CACHE when set. */
+ * for a batched schedule, time the batched coherence's rows per block (3 az
+ * x 10 rg window) and the burst group count, keeping the fastest of each.
+ * *best_strip_rows gets the chosen strip_rows value.  Cached in the file
+ * named by TOPS_TUNE_CACHE when set. */
 int tops_plan_tune(tops_plan* plan, void* stream, int* best_strip_rows);
 int tops_plan_strip_rows(const tops_plan* plan);
 /* Rows per block of the plan's batched coherence walk (default 32). */
 int tops_plan_coh_rows(const tops_plan* plan);
+/* Burst groups of the batched schedule (default 1): the bursts split into
+ * that many runs of consecutive bursts with about equal sample counts; each
+ * group is resampled by one launch on the caller's stream and its coherence
+ * launch, on a second stream of the plan, waits for that launch only, so it
+ * runs beside the next group's resampling.  The plan's run (and its graph)
+ * ends with the caller's stream waiting for the second stream. */
+int tops_plan_set_groups(tops_plan* plan, int groups);
+int tops_plan_groups(const tops_plan* plan);
 /* The candidates the last tops_plan_tune timed, in order: entry i gives
- * its strip_rows, coherence rows per block and mean ms per run. */
+ * its strip_rows, coherence rows per block, burst groups and mean ms per
+ * run. */
 int tops_plan_tune_count(const tops_plan* plan);
 int tops_plan_tune_entry(const tops_plan* plan, int i, int* strip_rows, int* coh_rows,
-                         float* ms);
+                         int* groups, float* ms);
 /* Durations of the plan's launches: `reps` eager runs of its schedule on
  * `stream` with an event pair around every launch.  ms[k] (k = 0 resample,
  * 1 coherence, 2 fused resample + coherence; 3 entries) gets the mean per

@@ -125,6 +125,9 @@ int check_burst_geom(const tops_burst_geom* g, const char* who)
     if (g->master_box[2] <= -kMaxSlaveLine || g->master_box[3] >= kMaxSlaveLine)
         return set_error(TOPS_ERR_ARG, "%s: master lines [%d,%d] outside +-%d", who,
                          g->master_box[2], g->master_box[3], kMaxSlaveLine);
+    if ((long long)(g->slave_box[3] - g->slave_box[2] + 1) *
+            (g->slave_box[1] - g->slave_box[0] + 1) * 4 > kMaxSlaveBytes)
+        return set_error(TOPS_ERR_ARG, "%s: slave box over 2^31 bytes of CInt16", who);
     return TOPS_OK;
 }
 

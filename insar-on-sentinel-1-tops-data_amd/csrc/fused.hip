@@ -139,8 +139,7 @@ __device__ __forceinline__ void fused_item(CParams* it, const tops_ci16* cm, flo
     // (computed in vector registers; the same in every lane, so kept scalar)
     const int ucol0 = __builtin_amdgcn_readfirstlane(ix0 - kHalf - 1 - (cv - c_first));
     ColumnWalker w;
-    w.xcol = ucol0 + g;
-    w.in_box = w.xcol >= p.sb[0] && w.xcol <= p.sb[1];
+    w.init(p, ucol0 + g);
     {
         double a = 0.0, b = 0.0, cc = 0.0;
         if (w.in_box) phase_coeffs(p, w.xcol, a, b, cc);
@@ -155,7 +154,7 @@ __device__ __forceinline__ void fused_item(CParams* it, const tops_ci16* cm, flo
     }
 
     const double azs = az_shift_of(p);
-    float2 win[WK_WIN];
+    WinRow win[WK_WIN];
     int wbase = INT_MIN;   // absolute slave line held in win[0]
     float4 pm1 = make_float4(0.f, 0.f, 0.f, 0.f);   // products of row t-1
     float4 pm2 = pm1;                                 // products of row t-2
@@ -273,16 +272,12 @@ __device__ __forceinline__ void fused_item(CParams* it, const tops_ci16* cm, flo
 #pragma unroll
                     for (int q = 0; q < WK_WIN; q++) raw[q] = w.fetch(p, wbase + q);
 #pragma unroll
-                    for (int q = 0; q < WK_WIN; q++) win[q] = w.deramp(raw[q]);
+                    for (int q = 0; q < WK_WIN; q++) win[q] = win_pack(w.deramp(raw[q]));
                     w.prefetch(p);
                     o_win = 0;
                 }
                 while (o_win > 3) {   // slide by 4 rows
-#pragma unroll
-                    for (int q = 0; q < WK_WIN - 4; q++) win[q] = win[q + 4];
-#pragma unroll
-                    for (int q = 0; q < 4; q++) win[WK_WIN - 4 + q] = w.deramp(w.pend[q]);
-                    w.prefetch(p);
+                    slide4(win, w, p);
                     wbase += 4;
                     o_win -= 4;
                 }

@@ -2,10 +2,12 @@
 // of an IW pair, or a stack of slaves against one master): one strip table
 // covers every burst, so the whole deramp -> resample -> reramp pass is one
 // launch and the coherence pass another; the pair of launches can be
-// captured once in a hipGraph and replayed.  A third schedule fuses both
-// passes in one persistent launch.  The schedule, its strip height or row
-// band, and the coherence walk's rows per block are tuning parameters chosen
-// by timing the candidates on the plan's own bursts.
+// captured once in a hipGraph and replayed.  The bursts can be split into
+// groups, each group's coherence launch running on a second stream beside
+// the next group's resampling.  A third schedule fuses both passes in one
+// persistent launch.  The schedule, its strip height or row band, the
+// coherence walk's rows per block and the group count are tuning parameters
+// chosen by timing the candidates on the plan's own bursts.
 #include <algorithm>
 #include <stdio.h>
 #include <stdlib.h>
@@ -43,9 +45,21 @@ struct tops_plan {
     bool coh_batched = true;
     bool coh_walk = false;     // the window is the walking form's (3 az x 10 rg)
     int coh_rows = kDefaultCohRows;   // output rows per block of the batched coherence walk
-    // the tuner's record: every candidate it timed (strip_rows, coh_rows, ms
-    // per run of the plan)
-    struct TuneEntry { int strip_rows, coh_rows; float ms; };
+    // batched schedule in burst groups: group g (items grp_first[g] ..
+    // grp_first[g+1]-1) is resampled by one launch on the caller's stream,
+    // and its coherence launch on a second stream waits for that launch
+    // only, so it runs beside the next group's resampling.  groups = 1 is
+    // the plain two-launch schedule.
+    int groups = 1;
+    std::vector<int> grp_first;              // G + 1 item indices
+    std::vector<int> rs_off, coh_off;        // G + 1 offsets into the two work tables
+    std::vector<int> coh_item;               // item index of each coherence item
+    hipStream_t s2 = nullptr;                // the coherence stream (default priority)
+    std::vector<hipEvent_t> ev_grp;          // per group: its resample launch is done
+    hipEvent_t ev_join = nullptr;            // the coherence stream's work is done
+    // the tuner's record: every candidate it timed (strip_rows, coh_rows,
+    // groups, ms per run of the plan)
+    struct TuneEntry { int strip_rows, coh_rows, groups; float ms; };
     std::vector<TuneEntry> tune_log;
     // fused schedule: coherence item of each burst (-1: none), the work
     // table and the queue counters of the persistent launch
@@ -62,6 +76,44 @@ struct tops_plan {
 namespace {
 
 int fill_coh_work(tops_plan* p);
+
+// groups of consecutive bursts with about equal sample counts
+void partition(tops_plan* p)
+{
+    const int n = (int)p->items.size();
+    const int G = std::max(1, std::min(p->groups, n));
+    p->groups = G;
+    std::vector<double> cum(n + 1, 0.0);
+    for (int i = 0; i < n; i++) {
+        const tops_burst_geom& g = p->items[i].geom;
+        cum[i + 1] = cum[i] + (double)(g.master_box[3] - g.master_box[2] + 1) *
+                                  (g.master_box[1] - g.master_box[0] + 1);
+    }
+    p->grp_first.assign(G + 1, n);
+    p->grp_first[0] = 0;
+    for (int g = 1; g < G; g++) {
+        // the first burst at or past g/G of the samples, with at least one
+        // burst in this group and one left for each later group
+        const double target = cum[n] * g / G;
+        int j = p->grp_first[g - 1] + 1;
+        while (j < n - (G - g) && cum[j] < target) j++;
+        p->grp_first[g] = j;
+    }
+}
+
+// the second stream and the events of a grouped schedule
+int ensure_pipe(tops_plan* p)
+{
+    if (p->groups <= 1) return TOPS_OK;
+    if (!p->s2) TOPS_HIP(hipStreamCreate(&p->s2));
+    if (!p->ev_join) TOPS_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+    while ((int)p->ev_grp.size() < p->groups) {
+        hipEvent_t e;
+        TOPS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        p->ev_grp.push_back(e);
+    }
+    return TOPS_OK;
+}
 
 int fill_params(tops_plan* p)
 {
@@ -116,6 +168,10 @@ int fill_params(tops_plan* p)
         ci.push_back(c);
     }
     p->h_coh = ci;
+    p->coh_item.assign(ci.size(), 0);
+    for (size_t i = 0; i < p->items.size(); i++)
+        if (p->coh_of[i] >= 0) p->coh_item[p->coh_of[i]] = (int)i;
+    partition(p);
     p->fused_ok = fused_window_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2,
                                     p->coh_win_rg, p->coh_win_az);
     if (p->fused_ok) {
@@ -146,19 +202,25 @@ int fill_coh_work(tops_plan* p)
     if (p->d_work_coh) (void)hipFree(p->d_work_coh);
     p->d_work_coh = nullptr;
     p->n_work_coh = 0;
+    p->coh_off.assign(p->groups + 1, 0);
     if (!p->coh_batched) return TOPS_OK;
     std::vector<int4> wc;
-    for (size_t k = 0; k < p->h_coh.size(); k++) {
-        const CohItem& c = p->h_coh[k];
-        int nx, ny, tag = 0;
-        if (p->coh_walk) {
-            coherence_walk_count(c.w, c.h, p->coh_rows, &nx, &ny);
-            tag = coherence_walk_vec(c.m, c.s, c.coh, c.w);
-        } else {
-            coherence_tile_count(c.w, c.h, &nx, &ny);
+    // group by group (coherence items are in burst order)
+    size_t k = 0;
+    for (int g = 0; g < p->groups; g++) {
+        for (; k < p->h_coh.size() && p->coh_item[k] < p->grp_first[g + 1]; k++) {
+            const CohItem& c = p->h_coh[k];
+            int nx, ny, tag = 0;
+            if (p->coh_walk) {
+                coherence_walk_count(c.w, c.h, p->coh_rows, &nx, &ny);
+                tag = coherence_walk_vec(c.m, c.s, c.coh, c.w);
+            } else {
+                coherence_tile_count(c.w, c.h, &nx, &ny);
+            }
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++) wc.push_back(make_int4((int)k, x, y, tag));
         }
-        for (int y = 0; y < ny; y++)
-            for (int x = 0; x < nx; x++) wc.push_back(make_int4((int)k, x, y, tag));
+        p->coh_off[g + 1] = (int)wc.size();
     }
     p->n_work_coh = (int)wc.size();
     if (!wc.empty()) {
@@ -168,8 +230,9 @@ int fill_coh_work(tops_plan* p)
     return TOPS_OK;
 }
 
-// strip table for the current strip height; strips of all bursts are
-// interleaved row-band by row-band so uneven bursts spread over the GPU
+// strip table for the current strip height; within each burst group the
+// strips of its bursts are interleaved row-band by row-band so uneven bursts
+// spread over the GPU
 int fill_work(tops_plan* p)
 {
     if (p->strip_rows < 0) {
@@ -200,18 +263,23 @@ int fill_work(tops_plan* p)
     }
     if (p->strip_rows == 0) return TOPS_OK;   // per-burst launches need no table
     std::vector<int4> w;
-    int max_ny = 0;
     std::vector<int> nxs(p->items.size()), nys(p->items.size());
     for (size_t i = 0; i < p->items.size(); i++) {
         const tops_burst_geom& g = p->items[i].geom;
         resample_strip_count(g.master_box[3] - g.master_box[2] + 1,
                              g.master_box[1] - g.master_box[0] + 1, p->strip_rows, &nxs[i], &nys[i]);
-        max_ny = std::max(max_ny, nys[i]);
     }
-    for (int y = 0; y < max_ny; y++)
-        for (size_t i = 0; i < p->items.size(); i++)
-            if (y < nys[i])
-                for (int x = 0; x < nxs[i]; x++) w.push_back(make_int4((int)i, x, y, p->strip_rows));
+    p->rs_off.assign(p->groups + 1, 0);
+    for (int gi = 0; gi < p->groups; gi++) {
+        const int i0 = p->grp_first[gi], i1 = p->grp_first[gi + 1];
+        int max_ny = 0;
+        for (int i = i0; i < i1; i++) max_ny = std::max(max_ny, nys[i]);
+        for (int y = 0; y < max_ny; y++)
+            for (int i = i0; i < i1; i++)
+                if (y < nys[i])
+                    for (int x = 0; x < nxs[i]; x++) w.push_back(make_int4(i, x, y, p->strip_rows));
+        p->rs_off[gi + 1] = (int)w.size();
+    }
     if (p->d_work_rs) (void)hipFree(p->d_work_rs);
     p->d_work_rs = nullptr;
     p->n_work_rs = (int)w.size();
@@ -289,6 +357,29 @@ int enqueue(tops_plan* p, hipStream_t s, Marks* mk = nullptr)
         }
         return rc;
     }
+    if (p->groups > 1 && p->coh_batched) {
+        // group g's coherence (second stream) waits for group g's resample
+        // launch only and runs beside the later groups' resampling
+        for (int g = 0; g < p->groups && !rc; g++) {
+            const int r0 = p->rs_off[g], nr = p->rs_off[g + 1] - r0;
+            const int c0 = p->coh_off[g], nc = p->coh_off[g + 1] - c0;
+            rc = marked(mk, s, kKindResample, [&] {
+                return launch_resample_batch(p->d_params, p->d_work_rs + r0, nr, s);
+            });
+            if (rc || nc == 0) continue;
+            TOPS_HIP(hipEventRecord(p->ev_grp[g], s));
+            TOPS_HIP(hipStreamWaitEvent(p->s2, p->ev_grp[g], 0));
+            rc = marked(mk, p->s2, kKindCoherence, [&] {
+                return launch_coherence_batch(p->d_coh, p->d_work_coh + c0, nc, dx, dy,
+                                              p->coh_win_rg, p->coh_win_az, p->coh_rows, p->s2);
+            });
+        }
+        // the caller's stream waits for the coherence stream (also when a
+        // launch failed: the fork is always joined)
+        TOPS_HIP(hipEventRecord(p->ev_join, p->s2));
+        TOPS_HIP(hipStreamWaitEvent(s, p->ev_join, 0));
+        return rc;
+    }
     rc = marked(mk, s, kKindResample,
                 [&] { return launch_resample_batch(p->d_params, p->d_work_rs, p->n_work_rs, s); });
     if (rc) return rc;
@@ -362,6 +453,9 @@ void tops_plan_destroy(tops_plan* p)
     if (p->d_coh_of) (void)hipFree(p->d_coh_of);
     if (p->d_work_fused) (void)hipFree(p->d_work_fused);
     if (p->d_queue) (void)hipFree(p->d_queue);
+    for (hipEvent_t e : p->ev_grp) (void)hipEventDestroy(e);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->s2) (void)hipStreamDestroy(p->s2);
     delete p;
 }
 
@@ -397,6 +491,20 @@ int tops_plan_capture_graph(tops_plan* p, void* stream)
 }
 
 int tops_plan_strip_rows(const tops_plan* p) { return p ? p->strip_rows : 0; }
+
+int tops_plan_set_groups(tops_plan* p, int groups)
+{
+    if (!p || groups <= 0) return set_error(TOPS_ERR_ARG, "tops_plan_set_groups: bad arguments");
+    drop_graph(p);
+    p->groups = groups;
+    partition(p);
+    int rc = ensure_pipe(p);
+    if (!rc) rc = fill_work(p);
+    if (!rc) rc = fill_coh_work(p);
+    return rc;
+}
+
+int tops_plan_groups(const tops_plan* p) { return p ? p->groups : 0; }
 
 // Launch durations of the plan's schedule: `reps` eager runs on `stream`,
 // an event pair around every launch; ms[k] = the mean per run of the
@@ -450,18 +558,21 @@ const char* tops_plan_kernel_name(const tops_plan* p, int kind)
 
 }  // extern "C"
 
-// Autotuner, in two stages on the plan's own bursts (warm-up + 3 timed runs
-// per candidate, hipEvents on `stream`, resample and coherence together):
+// Autotuner, in three stages on the plan's own bursts (warm-up + 3 timed
+// runs per candidate, hipEvents on `stream`, resample and coherence
+// together):
 //   1. the schedule: per-burst launches, one batched launch at each
 //      candidate strip height, or the fused launch at each candidate row
 //      band (3 az x 10 rg window);
 //   2. for a batched schedule with the walking coherence (3 az x 10 rg), the
-//      coherence walk's rows per block.
+//      coherence walk's rows per block;
+//   3. for a batched schedule, the number of burst groups whose coherence
+//      runs on the second stream beside the next group's resampling.
 // It keeps the fastest of each stage and records every candidate timed
 // (tops_plan_tune_entry).  With TOPS_TUNE_CACHE set, the choice is read from
-// / appended to that text file ("<key> <strip_rows> <coh_rows>" lines; the
-// key names the burst count and coherence window and hashes the burst
-// shapes).
+// / appended to that text file ("<key> <strip_rows> <coh_rows> <groups>"
+// lines; the key names the burst count and coherence window and hashes the
+// burst shapes).
 namespace {
 
 // mean ms per run of the plan's current schedule over 3 runs after a warm-up
@@ -516,16 +627,19 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
         FILE* f = fopen(cache, "r");
         if (f) {
             char line[640], k[512];
-            int v, cr;
+            int v, cr, ng;
             while (fgets(line, sizeof(line), f)) {
-                if (sscanf(line, "%511s %d %d", k, &v, &cr) != 3) continue;
-                if (key == k && (v >= 0 || p->fused_ok) && cr > 0) {
+                if (sscanf(line, "%511s %d %d %d", k, &v, &cr, &ng) != 4) continue;
+                if (key == k && (v >= 0 || p->fused_ok) && cr > 0 && ng > 0) {
                     fclose(f);
                     drop_graph(p);
                     p->strip_rows = v;
                     p->coh_rows = cr;
+                    p->groups = ng;
+                    partition(p);
                     if (best_rows) *best_rows = v;
-                    int rc = fill_work(p);
+                    int rc = ensure_pipe(p);
+                    if (!rc) rc = fill_work(p);
                     return rc ? rc : fill_coh_work(p);
                 }
             }
@@ -550,6 +664,8 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
         }
     }
     p->coh_rows = kDefaultCohRows;
+    p->groups = 1;
+    partition(p);
     float best = 1e30f;
     int best_r = p->strip_rows;
     hipEvent_t e0, e1;
@@ -561,7 +677,7 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
         p->strip_rows = c;
         float ms = 0.f;
         if ((rc = fill_work(p)) || (rc = time_runs(p, s, e0, e1, &ms))) break;
-        p->tune_log.push_back({c, p->coh_rows, ms});
+        p->tune_log.push_back({c, p->coh_rows, p->groups, ms});
         if (ms < best) {
             best = ms;
             best_r = c;
@@ -576,7 +692,7 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
             p->coh_rows = cr;
             float ms = 0.f;
             if ((rc = fill_coh_work(p)) || (rc = time_runs(p, s, e0, e1, &ms))) break;
-            p->tune_log.push_back({best_r, cr, ms});
+            p->tune_log.push_back({best_r, cr, p->groups, ms});
             if (ms < best) {
                 best = ms;
                 best_c = cr;
@@ -585,12 +701,41 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
         p->coh_rows = best_c;
         if (!rc) rc = fill_coh_work(p);
     }
+    // stage 3: burst groups pipelined over two streams (batched schedule)
+    if (!rc && best_r > 0 && p->coh_batched && p->items.size() > 1) {
+        int best_g = 1;
+        std::vector<int> gs;
+        // (past about 6 groups a replayed graph of the plan measured slower
+        // than its eager run, 12.5 against 11.7 ms for 9 groups of the IW
+        // pair, so the search stops there)
+        for (int g : {2, 3, 4, 6})
+            if (g <= (int)p->items.size() && std::find(gs.begin(), gs.end(), g) == gs.end())
+                gs.push_back(g);
+        for (int g : gs) {
+            p->groups = g;
+            partition(p);
+            float ms = 0.f;
+            if ((rc = ensure_pipe(p)) || (rc = fill_work(p)) || (rc = fill_coh_work(p)) ||
+                (rc = time_runs(p, s, e0, e1, &ms)))
+                break;
+            p->tune_log.push_back({best_r, p->coh_rows, g, ms});
+            if (ms < best) {
+                best = ms;
+                best_g = g;
+            }
+        }
+        p->groups = best_g;
+        partition(p);
+        if (!rc) rc = ensure_pipe(p);
+        if (!rc) rc = fill_work(p);
+        if (!rc) rc = fill_coh_work(p);
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (!rc && cache) {
         FILE* f = fopen(cache, "a");
         if (f) {
-            fprintf(f, "%s %d %d\n", key.c_str(), p->strip_rows, p->coh_rows);
+            fprintf(f, "%s %d %d %d\n", key.c_str(), p->strip_rows, p->coh_rows, p->groups);
             fclose(f);
         }
     }
@@ -602,12 +747,14 @@ int tops_plan_coh_rows(const tops_plan* p) { return p ? p->coh_rows : 0; }
 
 int tops_plan_tune_count(const tops_plan* p) { return p ? (int)p->tune_log.size() : 0; }
 
-int tops_plan_tune_entry(const tops_plan* p, int i, int* strip_rows, int* coh_rows, float* ms)
+int tops_plan_tune_entry(const tops_plan* p, int i, int* strip_rows, int* coh_rows, int* groups,
+                         float* ms)
 {
     if (!p || i < 0 || i >= (int)p->tune_log.size())
         return set_error(TOPS_ERR_ARG, "tops_plan_tune_entry: no entry %d", i);
     if (strip_rows) *strip_rows = p->tune_log[i].strip_rows;
     if (coh_rows) *coh_rows = p->tune_log[i].coh_rows;
+    if (groups) *groups = p->tune_log[i].groups;
     if (ms) *ms = p->tune_log[i].ms;
     return TOPS_OK;
 }

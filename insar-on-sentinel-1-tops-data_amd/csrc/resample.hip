@@ -195,8 +195,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
         ucol0 = lo - kHalf;
     }
     ColumnWalker w;
-    w.xcol = ucol0 + lane;
-    w.in_box = w.xcol >= p.sb[0] && w.xcol <= p.sb[1];
+    w.init(p, ucol0 + lane);
     {
         double a = 0.0, b = 0.0, cc = 0.0;
         if (w.in_box) phase_coeffs(p, w.xcol, a, b, cc);
@@ -212,7 +211,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     }
 
     const double azs = az_shift_of(p);
-    float2 win[WK_WIN];
+    WinRow win[WK_WIN];
     int wbase = INT_MIN;   // absolute slave line held in win[0]
 
     for (int r = r0; r < r1; r++) {
@@ -287,16 +286,12 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
 #pragma unroll
                 for (int k = 0; k < WK_WIN; k++) raw[k] = w.fetch(p, wbase + k);
 #pragma unroll
-                for (int k = 0; k < WK_WIN; k++) win[k] = w.deramp(raw[k]);
+                for (int k = 0; k < WK_WIN; k++) win[k] = win_pack(w.deramp(raw[k]));
                 w.prefetch(p);
                 o_win = 0;
             }
             while (o_win > 3) {   // slide by 4 rows
-#pragma unroll
-                for (int k = 0; k < WK_WIN - 4; k++) win[k] = win[k + 4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) win[WK_WIN - 4 + k] = w.deramp(w.pend[k]);
-                w.prefetch(p);
+                slide4(win, w, p);
                 wbase += 4;
                 o_win -= 4;
             }
@@ -469,6 +464,9 @@ int launch_resample_rows(const ResampleParams& p, int row0, int rows, hipStream_
         return set_error(TOPS_ERR_ARG, "resample: rows %d .. %d outside the box's %d lines", row0,
                          row0 + rows - 1, p.m_lines);
     if (rows == 0) return TOPS_OK;
+    if ((long long)p.s_lines * p.s_pixels * 4 > kMaxSlaveBytes)
+        return set_error(TOPS_ERR_ARG, "resample: slave box of %d x %d samples over 2^31 bytes",
+                         p.s_lines, p.s_pixels);
     if (strip_rows <= 0) strip_rows = resample_rows_for_rounds(&rows, &p.m_pixels, 1, 1);
     dim3 grid((p.m_pixels + WK_OUT - 1) / WK_OUT, (rows + strip_rows - 1) / strip_rows);
     hipLaunchKernelGGL(resample_fast_kernel, grid, dim3(64), 0, s, p, strip_rows, row0,

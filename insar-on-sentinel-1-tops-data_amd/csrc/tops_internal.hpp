@@ -24,6 +24,8 @@ constexpr int kHalf = kTaps / 2 - 1;              // c_Npoints2m1
 // slave lines a burst geometry may address (0 .. kMaxSlaveLine - 1): the
 // resample walker packs (line, LUT row) into one int as line * 2048 + row
 constexpr int kMaxSlaveLine = (1 << 20) - 64;
+// the fast kernels address the slave raster with 32-bit buffer offsets
+constexpr long long kMaxSlaveBytes = (1ll << 31) - 1;
 
 // ---- status (thread-local, read through tops_last_error) ---------------
 int set_error(int code, const char* fmt, ...);

@@ -150,22 +150,45 @@ __device__ __forceinline__ tops_cf32 interior_direct(const tops_ci16* slave, con
 
 // Column state of one lane: deramp recurrence along its slave column, with
 // the raw samples of the next four lines already requested from HBM.
+// word 3 of a raw (untyped, unswizzled) buffer resource on gfx9 parts
+constexpr int kRawBufferWord3 = 0x00020000;
+
 struct ColumnWalker {
     int xcol;            // absolute slave column
     bool in_box;         // column inside the slave box
+    unsigned xoff;       // byte offset in a slave row of the column clamped into the box
     double pr, pi;       // exp(j phi(ynext))
     double ir, ii;       // exp(j (b + c (2 ynext + 1)))
     double qr, qi;       // exp(j 2c)
     int ynext;           // absolute slave line deramped next
     int pend[4];         // raw CInt16 of lines ynext .. ynext+3
 
-    // raw CInt16 (packed re | im << 16) of line y of this column, 0 outside
-    // the box (a zero sample deramps to zero)
+    // raw CInt16 of line y (inside the box) of this column: a buffer load
+    // with the line's byte offset as the scalar offset and the column's as
+    // the lane offset, so it takes no vector address arithmetic.  The
+    // launchers keep the slave raster under 2^31 bytes.
+    __device__ int load_line(const ResampleParams& p, int y) const
+    {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<tops_ci16*>(p.slave), 0, p.s_lines * p.s_pixels * 4, kRawBufferWord3);
+        return __builtin_amdgcn_raw_buffer_load_b32(r, (int)xoff, (y - p.sb[2]) * p.s_pixels * 4, 0);
+    }
+
+    __device__ void init(const ResampleParams& p, int col)
+    {
+        xcol = col;
+        in_box = col >= p.sb[0] && col <= p.sb[1];
+        xoff = 4u * (unsigned)(min(max(col, p.sb[0]), p.sb[1]) - p.sb[0]);
+    }
+
+    // raw CInt16 (packed re | im << 16) of line y of this column, 0 for a
+    // line outside the box (y is wave-uniform: a scalar branch).  A column
+    // outside the box loads its clamped neighbour; deramp() drops that
+    // value when it consumes it, so the load stays in flight until then.
     __device__ int fetch(const ResampleParams& p, int y) const
     {
-        if (!in_box || y < p.sb[2] || y > p.sb[3]) return 0;
-        return reinterpret_cast<const int*>(p.slave)[(size_t)(y - p.sb[2]) * p.s_pixels +
-                                                     (xcol - p.sb[0])];
+        if (y < p.sb[2] || y > p.sb[3]) return 0;
+        return load_line(p, y);
     }
 
     // phi(y) = a + b y + c y^2 (y burst-relative line), anchored at y_abs
@@ -182,10 +205,19 @@ struct ColumnWalker {
         ir = cc; ii = s;
     }
 
-    // deramped sample of line ynext from its raw value, then advance
-    __device__ float2 deramp(int raw)
+    // the raw value as floats (0 for a column outside the box)
+    __device__ float2 unpack(int raw) const
     {
-        const float re = (float)(short)(raw & 0xffff), im = (float)(short)(raw >> 16);
+        raw = in_box ? raw : 0;
+        return make_float2((float)(short)(raw & 0xffff), (float)(short)(raw >> 16));
+    }
+
+    // deramped sample of line ynext from its raw value, then advance
+    __device__ float2 deramp(int raw) { return deramp(unpack(raw)); }
+
+    __device__ float2 deramp(float2 v)
+    {
+        const float re = v.x, im = v.y;
         const float cr = (float)pr, ci = (float)pi;
         float2 d = make_float2(re * cr - im * ci, re * ci + im * cr);
         double nr = pr * ir - pi * ii, ni = pr * ii + pi * ir;
@@ -196,13 +228,54 @@ struct ColumnWalker {
         return d;
     }
 
-    // request the four lines after the ones deramped so far
-    __device__ void prefetch(const ResampleParams& p)
+    // request lines ynext + skip .. ynext + skip + 3 (one scalar test when
+    // all four lie inside the box): skip = 0 after the pending lines were
+    // deramped, 4 while they still wait
+    __device__ void prefetch(const ResampleParams& p, int skip = 0)
     {
+        const int y0 = ynext + skip;
+        if (y0 >= p.sb[2] && y0 + 3 <= p.sb[3]) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) pend[k] = fetch(p, ynext + k);
+            for (int k = 0; k < 4; k++) pend[k] = load_line(p, y0 + k);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) pend[k] = fetch(p, y0 + k);
+        }
     }
 };
+
+// A window row: one deramped sample as a 64-bit value (real part in the low
+// half).  Kept as one 64-bit register pair, the window's slide by 4 rows is
+// 12 64-bit moves instead of 24 32-bit ones.
+typedef unsigned long long WinRow;
+__device__ __forceinline__ WinRow win_pack(float2 v)
+{
+    return (WinRow)__float_as_uint(v.x) | ((WinRow)__float_as_uint(v.y) << 32);
+}
+__device__ __forceinline__ float win_re(WinRow w) { return __uint_as_float((unsigned)w); }
+__device__ __forceinline__ float win_im(WinRow w) { return __uint_as_float((unsigned)(w >> 32)); }
+
+// Slide a window by 4 rows: the 4 oldest rows out, the 4 pending rows
+// deramped in.  The raw values are unpacked and only then are the next 4
+// lines requested (the asm statement orders the two): the loads then land
+// in the registers the raw values held, and stay in flight until the next
+// slide.  (Requested earlier, the compiler copies the landing registers at
+// once, which waits for the loads.)
+template <typename Win>
+__device__ __forceinline__ void slide4(Win (&win)[WK_WIN], ColumnWalker& w,
+                                       const ResampleParams& p)
+{
+    float2 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = w.unpack(w.pend[k]);
+    asm volatile("" : "+v"(v[0].x), "+v"(v[0].y), "+v"(v[1].x), "+v"(v[1].y), "+v"(v[2].x),
+                 "+v"(v[2].y), "+v"(v[3].x), "+v"(v[3].y)::"memory");
+    w.prefetch(p, 4);   // the 4 lines after the ones unpacked
+#pragma unroll
+    for (int k = 0; k < WK_WIN - 4; k++) win[k] = win[k + 4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) win[WK_WIN - 4 + k] = win_pack(w.deramp(v[k]));
+}
 
 // read-only tables addressed with wave-uniform indices
 typedef __attribute__((address_space(4))) const float CFloat;
@@ -214,7 +287,7 @@ typedef __attribute__((address_space(4))) const double CDouble;
 // 12-term sum per offset behind a scalar branch measured 6% slower on the
 // IW pair.)
 static_assert(kAzShiftTaps == WK_WIN, "a shifted tap row spans the window");
-__device__ __forceinline__ float2 az_taps_shifted(const float2 (&win)[WK_WIN], const float* kaz_sh,
+__device__ __forceinline__ float2 az_taps_shifted(const WinRow (&win)[WK_WIN], const float* kaz_sh,
                                                   int oq, int fyq)
 {
     CFloat* taps = (CFloat*)(kaz_sh + ((size_t)oq * kLutRows + fyq) * kAzShiftTaps);
@@ -222,8 +295,8 @@ __device__ __forceinline__ float2 az_taps_shifted(const float2 (&win)[WK_WIN], c
 #pragma unroll
     for (int t = 0; t < kAzShiftTaps; t++) {
         const float k = taps[t];
-        u.x = fmaf(k, win[t].x, u.x);
-        u.y = fmaf(k, win[t].y, u.y);
+        u.x = fmaf(k, win_re(win[t]), u.x);
+        u.y = fmaf(k, win_im(win[t]), u.y);
     }
     return u;
 }

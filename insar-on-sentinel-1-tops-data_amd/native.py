@@ -36,6 +36,7 @@ EXPORTS = [
     "tops_plan_create", "tops_plan_run", "tops_plan_capture_graph", "tops_plan_tune",
     "tops_plan_strip_rows", "tops_plan_time_launches", "tops_plan_kernel_name",
     "tops_plan_coh_rows", "tops_plan_tune_count", "tops_plan_tune_entry",
+    "tops_plan_set_groups", "tops_plan_groups",
     "tops_plan_destroy",
     "tops_subswath_workspace_size", "tops_subswath_esd_coh",
     "tops_deburst_extent", "tops_deburst_rows", "tops_deburst", "tops_stitch_subswath",
@@ -261,7 +262,11 @@ def load():
     L.tops_plan_tune_count.restype = C.c_int
     L.tops_plan_tune_count.argtypes = [_vp]
     L.tops_plan_tune_entry.restype = C.c_int
-    L.tops_plan_tune_entry.argtypes = [_vp, C.c_int, _ip, _ip, _fp]
+    L.tops_plan_tune_entry.argtypes = [_vp, C.c_int, _ip, _ip, _ip, _fp]
+    L.tops_plan_set_groups.restype = C.c_int
+    L.tops_plan_set_groups.argtypes = [_vp, C.c_int]
+    L.tops_plan_groups.restype = C.c_int
+    L.tops_plan_groups.argtypes = [_vp]
     L.tops_plan_destroy.restype = None
     L.tops_plan_destroy.argtypes = [_vp]
     L.tops_subswath_workspace_size.restype = C.c_size_t

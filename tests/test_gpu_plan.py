@@ -131,16 +131,18 @@ def test_plan_tune_and_graph_replay(tops, dev, batch):
     assert n >= 10
     log = []
     for i in range(n):
-        r, cr, ms = C.c_int(), C.c_int(), C.c_float()
-        N.check(L.tops_plan_tune_entry(plan, i, C.byref(r), C.byref(cr), C.byref(ms)))
-        log.append((r.value, cr.value, ms.value))
-    assert all(ms > 0 for _, _, ms in log)
-    chosen = (best.value, L.tops_plan_coh_rows(plan))
+        r, cr, g, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        N.check(L.tops_plan_tune_entry(plan, i, C.byref(r), C.byref(cr), C.byref(g), C.byref(ms)))
+        log.append((r.value, cr.value, g.value, ms.value))
+    assert all(e[3] > 0 for e in log)
+    chosen = (best.value, L.tops_plan_coh_rows(plan), L.tops_plan_groups(plan))
     assert L.tops_plan_coh_rows(plan) in (16, 32, 64, 128, 256)
-    assert min(log, key=lambda e: e[2])[:2] == chosen, (log, chosen)
-    assert L.tops_plan_tune_entry(plan, n, None, None, None) != 0
+    assert 1 <= L.tops_plan_groups(plan) <= 5
+    assert min(log, key=lambda e: e[3])[:3] == chosen, (log, chosen)
+    assert L.tops_plan_tune_entry(plan, n, None, None, None, None) != 0
     L.tops_clear_error()
-    print("  plan: tuner " + ", ".join(f"{r}/{cr}: {ms:.3f}" for r, cr, ms in log), flush=True)
+    print("  plan: tuner " + ", ".join(f"{r}/{cr}/{g}: {ms:.3f}" for r, cr, g, ms in log),
+          flush=True)
     N.check(L.tops_plan_run(plan, sp))
     s.synchronize()
     eager = [(dev.get(d["out"], np.complex64, (b["L"], b["P"])).copy(),
@@ -158,6 +160,57 @@ def test_plan_tune_and_graph_replay(tops, dev, batch):
         for (b, d), (eo, ec) in zip(zip(batch["bursts"], batch["dev"]), eager):
             assert np.array_equal(dev.get(d["out"], np.complex64, (b["L"], b["P"])), eo), rep
             assert np.array_equal(dev.get(d["coh"], np.float32, (b["L"], b["P"])), ec), rep
+    L.tops_plan_destroy(plan)
+
+
+@pytest.mark.parametrize("groups", [2, 3, 5, 9])
+def test_plan_burst_groups(tops, dev, batch, groups):
+    """The batched schedule in burst groups (each group's coherence on the
+    plan's second stream, waiting for that group's resample launch only):
+    bit-identical to one group, eagerly on a caller stream and as a captured
+    hipGraph replayed 3 times; more groups than bursts give one per burst."""
+    N = tops.native
+    L = N.load()
+    torch = dev.torch
+    s = torch.cuda.Stream()
+    sp = C.c_void_p(s.cuda_stream)
+    plan = L.tops_plan_create(batch["ctx"], batch["items"], 5, 3, 10, 96)
+    assert plan, N.last_error()
+    N.check(L.tops_plan_run(plan, sp))
+    s.synchronize()
+    shp = [(b["L"], b["P"]) for b in batch["bursts"]]
+    one = [(dev.get(d["out"], np.complex64, sh).copy(), dev.get(d["coh"], np.float32, sh).copy())
+           for d, sh in zip(batch["dev"], shp)]
+    N.check(L.tops_plan_set_groups(plan, groups), "tops_plan_set_groups")
+    assert L.tops_plan_groups(plan) == min(groups, 5)
+    print(f"  plan: {L.tops_plan_groups(plan)} burst groups", flush=True)
+
+    def check(label):
+        for (d, sh), (eo, ec) in zip(zip(batch["dev"], shp), one):
+            assert np.array_equal(dev.get(d["out"], np.complex64, sh), eo), label
+            assert np.array_equal(dev.get(d["coh"], np.float32, sh), ec), label
+
+    for d in batch["dev"]:
+        d["out"].zero_()
+        d["coh"].zero_()
+    torch.cuda.synchronize()
+    N.check(L.tops_plan_run(plan, sp))
+    s.synchronize()
+    check("eager")
+    N.check(L.tops_plan_capture_graph(plan, sp), "tops_plan_capture_graph")
+    for rep in range(3):
+        for d in batch["dev"]:
+            d["out"].zero_()
+            d["coh"].zero_()
+        torch.cuda.synchronize()
+        N.check(L.tops_plan_run(plan, sp))
+        s.synchronize()
+        check(f"replay {rep}")
+        # per-launch timing between replays (eager launches of both streams)
+        ms = (C.c_float * 3)()
+        nl = (C.c_int * 3)()
+        N.check(L.tops_plan_time_launches(plan, sp, 1, ms, nl))
+        assert nl[0] == min(groups, 5) and nl[1] == min(groups, 5)
     L.tops_plan_destroy(plan)
 
 
