@@ -389,7 +389,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWalkWa
                             (int)(gridDim.x * gridDim.y));
     const int sx = t / (int)gridDim.y, sy = t - sx * (int)gridDim.y;
     const int r0 = row0 + sy * strip_rows;
-    resample_strip(p, sx, r0, min(r0 + strip_rows, row_end), sm);
+    // the parameter block read in place in the kernel-argument segment (it
+    // is the first argument): its fields then become scalar loads where they
+    // are used, as in the batched kernel, instead of SGPRs held for the whole
+    // kernel (which spill to VGPR lanes inside the row loop)
+    typedef __attribute__((address_space(4))) const ResampleParams CParams;
+    static_assert(offsetof(ResampleParams, mb) == 0, "parameter block layout");
+    CParams* pk = (CParams*)__builtin_amdgcn_kernarg_segment_ptr();
+    (void)p;
+    resample_strip(*(const ResampleParams*)pk, sx, r0, min(r0 + strip_rows, row_end), sm);
 }
 
 // Batched form: one workgroup per entry of a strip table covering many
