@@ -158,10 +158,34 @@ __device__ inline double wave_sum_d(double v)
     return v;
 }
 
-__global__ void __launch_bounds__(GEO_THREADS) geocoreg_kernel(
-    const int32_t* dem, int lines, int pixels, GeoRef g, int nodata, OrbitDev m, OrbitDev s,
-    int xmin, int xmax, int ymin, int ymax, double* partials, double* lp)
+// the kernel's arguments in one block, read in place in the kernel-argument
+// segment: the two orbits' coefficients become scalar loads where the
+// Newton solve uses them instead of SGPRs held for the whole kernel (which
+// spilled to VGPR lanes)
+struct GeoArgs {
+    const int32_t* dem;
+    int lines, pixels;
+    GeoRef g;
+    int nodata;
+    OrbitDev m, s;
+    int xmin, xmax, ymin, ymax;
+    double* partials;
+    double* lp;
+};
+
+__global__ void __launch_bounds__(GEO_THREADS) geocoreg_kernel(GeoArgs args)
 {
+    typedef __attribute__((address_space(4))) const GeoArgs CGeoArgs;
+    const GeoArgs& A = *(const GeoArgs*)(CGeoArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    (void)args;
+    const int32_t* dem = A.dem;
+    const int lines = A.lines, pixels = A.pixels, nodata = A.nodata;
+    const GeoRef& g = A.g;
+    const OrbitDev& m = A.m;
+    const OrbitDev& s = A.s;
+    const int xmin = A.xmin, xmax = A.xmax, ymin = A.ymin, ymax = A.ymax;
+    double* partials = A.partials;
+    double* lp = A.lp;
     __shared__ double red[GEO_THREADS / 64][kGeoSums];
     const long n = (long)lines * pixels;
     double acc[kGeoSums];
@@ -273,8 +297,21 @@ int launch_geocoreg(const int32_t* dem, int lines, int pixels, double lat0, doub
     g.dlon = dlon;
     g.e2 = 1.0 - (kEllB * kEllB / kEllA / kEllA);   // b_ell_e2, cuConstants.cuh:13
     g.one_e2 = 1.0 - g.e2;
-    hipLaunchKernelGGL(geocoreg_kernel, dim3(nblocks), dim3(GEO_THREADS), 0, st, dem, lines,
-                       pixels, g, nodata, m, s, xmin, xmax, ymin, ymax, d_partials, d_lp);
+    GeoArgs a;
+    a.dem = dem;
+    a.lines = lines;
+    a.pixels = pixels;
+    a.g = g;
+    a.nodata = nodata;
+    a.m = m;
+    a.s = s;
+    a.xmin = xmin;
+    a.xmax = xmax;
+    a.ymin = ymin;
+    a.ymax = ymax;
+    a.partials = d_partials;
+    a.lp = d_lp;
+    hipLaunchKernelGGL(geocoreg_kernel, dim3(nblocks), dim3(GEO_THREADS), 0, st, a);
     TOPS_HIP(hipGetLastError());
     hipLaunchKernelGGL(geocoreg_final_kernel, dim3(kGeoSums), dim3(256), 0, st, d_partials,
                        nblocks);
