@@ -220,16 +220,21 @@ def tuner_record(L, plan, chosen):
     coherence rows, one group)."""
     timed = []
     for i in range(L.tops_plan_tune_count(plan)):
-        r, cr, g, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
-        if L.tops_plan_tune_entry(plan, i, C.byref(r), C.byref(cr), C.byref(g), C.byref(ms)) == 0:
-            timed.append([r.value, cr.value, g.value, round(ms.value, 4)])
+        r, cr, g, st, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        if L.tops_plan_tune_entry(plan, i, C.byref(r), C.byref(cr), C.byref(g), C.byref(st),
+                                  C.byref(ms)) == 0:
+            timed.append([r.value, cr.value, g.value, st.value, round(ms.value, 4)])
     pick = [chosen, L.tops_plan_coh_rows(plan), L.tops_plan_groups(plan)]
-    ms_of = {(r, c, g): m for r, c, g, m in timed}
-    default = ms_of.get((0, 32, 1))
-    tuned = ms_of.get(tuple(pick))
-    return {"timed": timed, "chosen": pick, "chosen_ms": tuned, "default": [0, 32, 1],
+    # eager timings (stages 1 and 2) of the default and of the chosen schedule
+    # with one group
+    eager = {(r, c, g): m for r, c, g, st, m in timed if st in (1, 2)}
+    default = eager.get((0, 32, 1))
+    tuned = eager.get((pick[0], pick[1], 1))
+    return {"timed": timed, "chosen": pick, "chosen_ms_one_group": tuned, "default": [0, 32, 1],
             "default_ms": default,
-            "speedup_vs_default": round(default / tuned, 3) if default and tuned else None}
+            "speedup_vs_default": round(default / tuned, 3) if default and tuned else None,
+            "note": "entries [strip_rows, coh_rows, groups, stage, ms]; stages 1 and 2 eager "
+                    "runs, stage 3 (burst groups) replayed graphs"}
 
 
 def issue_fields(roof, kernel, workload, launch_s, samples_per_launch):
@@ -335,7 +340,8 @@ def run_batched(args, tops, torch):
     tuner = tuner_record(L, plan, best.value)
     if args.groups > 0:
         N.check(L.tops_plan_set_groups(plan, args.groups), "groups")
-    print(f"[bench] {workload}: tuner " + ", ".join(f"{r}/{c}/{g}: {m}" for r, c, g, m in tuner["timed"]),
+    print(f"[bench] {workload}: tuner " + ", ".join(f"{r}/{c}/{g}/s{st}: {m}"
+                                                     for r, c, g, st, m in tuner["timed"]),
           file=sys.stderr, flush=True)
     names = [L.tops_plan_kernel_name(plan, k).decode() for k in range(3)]
     N.check(L.tops_plan_capture_graph(plan, sp), "graph")

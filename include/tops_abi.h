@@ -236,7 +236,8 @@ int tops_plan_capture_graph(tops_plan* plan, void* stream);
 /* Time the candidate schedules on this plan (per-burst, batched at several
  * strip heights, fused at several row bands) and keep the fastest; then,
  * for a batched schedule, time the batched coherence's rows per block (3 az
- * x 10 rg window) and the burst group count, keeping the fastest of each.
+ * x 10 rg window) and the burst group count (as replayed graphs), keeping
+ * the fastest of each.
  * *best_strip_rows gets the chosen strip_rows value.  Cached in the file
  * named by TOPS_TUNE_CACHE when set. */
 int tops_plan_tune(tops_plan* plan, void* stream, int* best_strip_rows);
@@ -252,11 +253,13 @@ int tops_plan_coh_rows(const tops_plan* plan);
 int tops_plan_set_groups(tops_plan* plan, int groups);
 int tops_plan_groups(const tops_plan* plan);
 /* The candidates the last tops_plan_tune timed, in order: entry i gives
- * its strip_rows, coherence rows per block, burst groups and mean ms per
- * run. */
+ * its strip_rows, coherence rows per block, burst groups, tuning stage (1
+ * schedule, 2 coherence rows, 3 burst groups) and mean ms per run (eager
+ * runs in stages 1 and 2; replays of a captured graph in stage 3 unless the
+ * stream was the null stream). */
 int tops_plan_tune_count(const tops_plan* plan);
 int tops_plan_tune_entry(const tops_plan* plan, int i, int* strip_rows, int* coh_rows,
-                         int* groups, float* ms);
+                         int* groups, int* stage, float* ms);
 /* Durations of the plan's launches: `reps` eager runs of its schedule on
  * `stream` with an event pair around every launch.  ms[k] (k = 0 resample,
  * 1 coherence, 2 fused resample + coherence; 3 entries) gets the mean per

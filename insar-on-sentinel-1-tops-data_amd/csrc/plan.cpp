@@ -59,7 +59,7 @@ struct tops_plan {
     hipEvent_t ev_join = nullptr;            // the coherence stream's work is done
     // the tuner's record: every candidate it timed (strip_rows, coh_rows,
     // groups, ms per run of the plan)
-    struct TuneEntry { int strip_rows, coh_rows, groups; float ms; };
+    struct TuneEntry { int strip_rows, coh_rows, groups, stage; float ms; };
     std::vector<TuneEntry> tune_log;
     // fused schedule: coherence item of each burst (-1: none), the work
     // table and the queue counters of the persistent launch
@@ -567,7 +567,8 @@ const char* tops_plan_kernel_name(const tops_plan* p, int kind)
 //   2. for a batched schedule with the walking coherence (3 az x 10 rg), the
 //      coherence walk's rows per block;
 //   3. for a batched schedule, the number of burst groups whose coherence
-//      runs on the second stream beside the next group's resampling.
+//      runs on the second stream beside the next group's resampling (one
+//      group included), timed as replayed graphs.
 // It keeps the fastest of each stage and records every candidate timed
 // (tops_plan_tune_entry).  With TOPS_TUNE_CACHE set, the choice is read from
 // / appended to that text file ("<key> <strip_rows> <coh_rows> <groups>"
@@ -594,6 +595,35 @@ int time_runs(tops_plan* p, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* 
     }
     *out = tot / 3.f;
     return TOPS_OK;
+}
+
+// mean ms per replay of a graph of the plan's current schedule, captured on
+// s (not the null stream), over 3 replays after a warm-up
+int time_graph(tops_plan* p, hipStream_t s, hipEvent_t e0, hipEvent_t e1, float* out)
+{
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    TOPS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue(p, s);
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "tops_plan_tune capture", __FILE__, __LINE__);
+    if (!rc && (e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0)) != hipSuccess)
+        rc = hip_fail(e, "tops_plan_tune instantiate", __FILE__, __LINE__);
+    float tot = 0.f;
+    for (int k = -1; k < 3 && !rc; k++) {   // k = -1: warm-up
+        e = hipEventRecord(e0, s);
+        if (e == hipSuccess) e = hipGraphLaunch(x, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) rc = hip_fail(e, "tops_plan_tune graph timing", __FILE__, __LINE__);
+        if (k >= 0) tot += ms;
+    }
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    if (!rc) *out = tot / 3.f;
+    return rc;
 }
 
 }  // namespace
@@ -677,7 +707,7 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
         p->strip_rows = c;
         float ms = 0.f;
         if ((rc = fill_work(p)) || (rc = time_runs(p, s, e0, e1, &ms))) break;
-        p->tune_log.push_back({c, p->coh_rows, p->groups, ms});
+        p->tune_log.push_back({c, p->coh_rows, p->groups, 1, ms});
         if (ms < best) {
             best = ms;
             best_r = c;
@@ -692,7 +722,7 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
             p->coh_rows = cr;
             float ms = 0.f;
             if ((rc = fill_coh_work(p)) || (rc = time_runs(p, s, e0, e1, &ms))) break;
-            p->tune_log.push_back({best_r, cr, p->groups, ms});
+            p->tune_log.push_back({best_r, cr, p->groups, 2, ms});
             if (ms < best) {
                 best = ms;
                 best_c = cr;
@@ -701,14 +731,17 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
         p->coh_rows = best_c;
         if (!rc) rc = fill_coh_work(p);
     }
-    // stage 3: burst groups pipelined over two streams (batched schedule)
+    // stage 3: burst groups pipelined over two streams (batched schedule),
+    // one group included, timed as replayed graphs when the caller's stream
+    // can capture one (the overlap of the two streams is not the same in a
+    // graph as in eager launches: 9 groups of the IW pair measured 11.7 ms
+    // eager and 12.5 ms replayed, so graphs decide, and the search stops at
+    // 6 groups)
     if (!rc && best_r > 0 && p->coh_batched && p->items.size() > 1) {
         int best_g = 1;
+        float best3 = 1e30f;
         std::vector<int> gs;
-        // (past about 6 groups a replayed graph of the plan measured slower
-        // than its eager run, 12.5 against 11.7 ms for 9 groups of the IW
-        // pair, so the search stops there)
-        for (int g : {2, 3, 4, 6})
+        for (int g : {1, 2, 3, 4, 6})
             if (g <= (int)p->items.size() && std::find(gs.begin(), gs.end(), g) == gs.end())
                 gs.push_back(g);
         for (int g : gs) {
@@ -716,11 +749,11 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
             partition(p);
             float ms = 0.f;
             if ((rc = ensure_pipe(p)) || (rc = fill_work(p)) || (rc = fill_coh_work(p)) ||
-                (rc = time_runs(p, s, e0, e1, &ms)))
+                (rc = s ? time_graph(p, s, e0, e1, &ms) : time_runs(p, s, e0, e1, &ms)))
                 break;
-            p->tune_log.push_back({best_r, p->coh_rows, g, ms});
-            if (ms < best) {
-                best = ms;
+            p->tune_log.push_back({best_r, p->coh_rows, g, 3, ms});
+            if (ms < best3) {
+                best3 = ms;
                 best_g = g;
             }
         }
@@ -748,13 +781,14 @@ int tops_plan_coh_rows(const tops_plan* p) { return p ? p->coh_rows : 0; }
 int tops_plan_tune_count(const tops_plan* p) { return p ? (int)p->tune_log.size() : 0; }
 
 int tops_plan_tune_entry(const tops_plan* p, int i, int* strip_rows, int* coh_rows, int* groups,
-                         float* ms)
+                         int* stage, float* ms)
 {
     if (!p || i < 0 || i >= (int)p->tune_log.size())
         return set_error(TOPS_ERR_ARG, "tops_plan_tune_entry: no entry %d", i);
     if (strip_rows) *strip_rows = p->tune_log[i].strip_rows;
     if (coh_rows) *coh_rows = p->tune_log[i].coh_rows;
     if (groups) *groups = p->tune_log[i].groups;
+    if (stage) *stage = p->tune_log[i].stage;
     if (ms) *ms = p->tune_log[i].ms;
     return TOPS_OK;
 }

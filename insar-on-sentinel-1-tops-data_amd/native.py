@@ -262,7 +262,7 @@ def load():
     L.tops_plan_tune_count.restype = C.c_int
     L.tops_plan_tune_count.argtypes = [_vp]
     L.tops_plan_tune_entry.restype = C.c_int
-    L.tops_plan_tune_entry.argtypes = [_vp, C.c_int, _ip, _ip, _ip, _fp]
+    L.tops_plan_tune_entry.argtypes = [_vp, C.c_int, _ip, _ip, _ip, _ip, _fp]
     L.tops_plan_set_groups.restype = C.c_int
     L.tops_plan_set_groups.argtypes = [_vp, C.c_int]
     L.tops_plan_groups.restype = C.c_int

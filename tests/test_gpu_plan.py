@@ -131,18 +131,30 @@ def test_plan_tune_and_graph_replay(tops, dev, batch):
     assert n >= 10
     log = []
     for i in range(n):
-        r, cr, g, ms = C.c_int(), C.c_int(), C.c_int(), C.c_float()
-        N.check(L.tops_plan_tune_entry(plan, i, C.byref(r), C.byref(cr), C.byref(g), C.byref(ms)))
-        log.append((r.value, cr.value, g.value, ms.value))
-    assert all(e[3] > 0 for e in log)
-    chosen = (best.value, L.tops_plan_coh_rows(plan), L.tops_plan_groups(plan))
-    assert L.tops_plan_coh_rows(plan) in (16, 32, 64, 128, 256)
-    assert 1 <= L.tops_plan_groups(plan) <= 5
-    assert min(log, key=lambda e: e[3])[:3] == chosen, (log, chosen)
-    assert L.tops_plan_tune_entry(plan, n, None, None, None, None) != 0
-    L.tops_clear_error()
-    print("  plan: tuner " + ", ".join(f"{r}/{cr}/{g}: {ms:.3f}" for r, cr, g, ms in log),
+        r, cr, g, st, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        N.check(L.tops_plan_tune_entry(plan, i, C.byref(r), C.byref(cr), C.byref(g), C.byref(st),
+                                       C.byref(ms)))
+        log.append((r.value, cr.value, g.value, st.value, ms.value))
+    print("  plan: tuner " + ", ".join(f"{r}/{cr}/{g}/s{st}: {ms:.3f}" for r, cr, g, st, ms in log),
           flush=True)
+    assert all(e[4] > 0 for e in log)
+    coh, groups = L.tops_plan_coh_rows(plan), L.tops_plan_groups(plan)
+    assert coh in (16, 32, 64, 128, 256) and 1 <= groups <= 5
+    ms = lambda e: e[4]
+    # each stage keeps its fastest candidate: the schedule among stage 1,
+    # the coherence rows among stage 2 and the stage-1 best, the groups
+    # among stage 3 (replayed graphs, one group included)
+    s1 = [e for e in log if e[3] == 1]
+    b1 = min(s1, key=ms)
+    assert best.value == b1[0], log
+    if best.value > 0:
+        s2 = [b1] + [e for e in log if e[3] == 2]
+        assert coh == min(s2, key=ms)[1], log
+        s3 = [e for e in log if e[3] == 3]
+        assert s3 and all(e[:2] == (best.value, coh) for e in s3), log
+        assert groups == min(s3, key=ms)[2], log
+    assert L.tops_plan_tune_entry(plan, n, None, None, None, None, None) != 0
+    L.tops_clear_error()
     N.check(L.tops_plan_run(plan, sp))
     s.synchronize()
     eager = [(dev.get(d["out"], np.complex64, (b["L"], b["P"])).copy(),
