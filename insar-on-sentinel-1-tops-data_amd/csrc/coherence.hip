@@ -53,6 +53,8 @@ __global__ void __launch_bounds__(256) coherence_fp64_kernel(const tops_ci16* M,
     coh[q0] = pp > 0.0 ? (float)(sqrt(a0 * a0 + a1 * a1) / sqrt(pp)) : 0.0f;
 }
 
+// CWR x CWA: a window fixed at compile time (0 x 0: win_rg x win_az)
+template <int CWR = 0, int CWA = 0>
 __device__ __forceinline__ void coherence_tile(const tops_ci16* M, const tops_cf32* S, int w,
                                                int h, int dx, int dy, int win_rg, int win_az,
                                                float* coh, int tx, int ty, float4* smem)
@@ -60,8 +62,9 @@ __device__ __forceinline__ void coherence_tile(const tops_ci16* M, const tops_cf
     const int r0 = ty * TW_ROWS, c0 = tx * TW_COLS;
     WindowSums ws[4];
     // (uniform branch; the launch's LDS is tile_window_launch_lds for it)
-    if (tile_window_shared_ok(win_rg, win_az))
-        tile_window_sums_shared(M, S, w, h, r0, c0, dx, dy, win_rg, win_az, smem, ws, nullptr);
+    if (CWR ? true : tile_window_shared_ok(win_rg, win_az))
+        tile_window_sums_shared<CWR, CWA>(M, S, w, h, r0, c0, dx, dy, win_rg, win_az, smem, ws,
+                                          nullptr);
     else
         tile_window_sums(M, S, w, h, r0, c0, dx, dy, win_rg, win_az, smem, ws, nullptr);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -79,23 +82,31 @@ __device__ __forceinline__ void coherence_tile(const tops_ci16* M, const tops_cf
     }
 }
 
+template <int CWR = 0, int CWA = 0>
 __global__ void __launch_bounds__(256) coherence_fast_kernel(const tops_ci16* M, const tops_cf32* S, int w,
                                       int h, int dx, int dy, int win_rg, int win_az,
                                       float* coh)
 {
     extern __shared__ float4 smem[];
-    coherence_tile(M, S, w, h, dx, dy, win_rg, win_az, coh, blockIdx.x, blockIdx.y, smem);
+    coherence_tile<CWR, CWA>(M, S, w, h, dx, dy, win_rg, win_az, coh, blockIdx.x, blockIdx.y,
+                             smem);
 }
 
 // Batched form: work[i] = {item, tile x, tile y, 0}.
+template <int CWR = 0, int CWA = 0>
 __global__ void __launch_bounds__(256) coherence_batch_kernel(const CohItem* items, const int4* work,
                                                               int dx, int dy, int win_rg, int win_az)
 {
     extern __shared__ float4 smem[];
     const int4 wk = work[blockIdx.x];
     const CohItem& it = items[wk.x];
-    coherence_tile(it.m, it.s, it.w, it.h, dx, dy, win_rg, win_az, it.coh, wk.y, wk.z, smem);
+    coherence_tile<CWR, CWA>(it.m, it.s, it.w, it.h, dx, dy, win_rg, win_az, it.coh, wk.y, wk.z,
+                             smem);
 }
+
+// the reference's ESDAndCoh window (6 az x 24 rg) takes kernels with the
+// window fixed at compile time, its loops unrolled
+static bool reference_window(int win_rg, int win_az) { return win_rg == 24 && win_az == 6; }
 
 // Walking form for the 3 az x 10 rg window centred as the stage layer
 // centres it (dx = 4, dy = 1): 256 threads = 4 waves, 2 columns per
@@ -216,8 +227,12 @@ int launch_coherence_batch(const CohItem* d_items, const int4* d_work, int n_wor
     if (lds == 0)
         return set_error(TOPS_ERR_ARG, "batched coherence: window %dx%d exceeds the LDS tile",
                          win_az, win_rg);
-    hipLaunchKernelGGL(coherence_batch_kernel, dim3(n_work), dim3(256), lds, st, d_items, d_work,
-                       dx, dy, win_rg, win_az);
+    if (reference_window(win_rg, win_az))
+        hipLaunchKernelGGL((coherence_batch_kernel<24, 6>), dim3(n_work), dim3(256), lds, st,
+                           d_items, d_work, dx, dy, win_rg, win_az);
+    else
+        hipLaunchKernelGGL(coherence_batch_kernel, dim3(n_work), dim3(256), lds, st, d_items,
+                           d_work, dx, dy, win_rg, win_az);
     TOPS_HIP(hipGetLastError());
     return TOPS_OK;
 }
@@ -237,9 +252,14 @@ int launch_coherence(const tops_ci16* m, const tops_cf32* s, int w, int h, int d
                            h, coh, coherence_walk_vec(m, s, coh, w), rows);
     } else {
         dim3 g((w + TW_COLS - 1) / TW_COLS, (h + TW_ROWS - 1) / TW_ROWS);
-        hipLaunchKernelGGL(coherence_fast_kernel, g, dim3(256),
-                           tile_window_launch_lds(win_rg, win_az), st, m, s, w, h, dx, dy,
-                           win_rg, win_az, coh);
+        if (reference_window(win_rg, win_az))
+            hipLaunchKernelGGL((coherence_fast_kernel<24, 6>), g, dim3(256),
+                               tile_window_launch_lds(win_rg, win_az), st, m, s, w, h, dx, dy,
+                               win_rg, win_az, coh);
+        else
+            hipLaunchKernelGGL(coherence_fast_kernel, g, dim3(256),
+                               tile_window_launch_lds(win_rg, win_az), st, m, s, w, h, dx, dy,
+                               win_rg, win_az, coh);
     }
     TOPS_HIP(hipGetLastError());
     return TOPS_OK;

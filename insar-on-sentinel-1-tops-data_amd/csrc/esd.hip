@@ -73,8 +73,10 @@ __device__ inline float wave_sum(float v)
     return v;
 }
 
-template <bool kShared>
-__global__ void __launch_bounds__(256) esd_band_kernel(EsdParams p)
+// kShared: the shared window sums (tile_window_sums_shared); CWR x CWA: a
+// window fixed at compile time (0 x 0: the parameter block's)
+template <bool kShared, int CWR = 0, int CWA = 0>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) esd_band_kernel(EsdParams p)
 {
     extern __shared__ float4 smem[];
     __shared__ float red[4][2];
@@ -92,7 +94,8 @@ __global__ void __launch_bounds__(256) esd_band_kernel(EsdParams p)
     }
     const size_t off_f = (size_t)p.band[b].f_offset * p.pixels;
     const size_t off_b = (size_t)p.band[b].b_offset * p.pixels;
-    const int dh = (p.win_az - 1) / 2, dw = (p.win_rg - 1) / 2;
+    const int win_az = CWA ? CWA : p.win_az, win_rg = CWR ? CWR : p.win_rg;
+    const int dh = (win_az - 1) / 2, dw = (win_rg - 1) / 2;
     // the two bands in turn through one copy of the window code (a loop kept
     // rolled, so both share its registers): coherence and centre product of
     // each output sample
@@ -105,10 +108,10 @@ __global__ void __launch_bounds__(256) esd_band_kernel(EsdParams p)
         WindowSums ws[4];
         float2 c[4];
         if (kShared)
-            tile_window_sums_shared(M, S, p.pixels, n, r0, c0, dw, dh, p.win_rg, p.win_az, smem,
-                                    ws, c);
+            tile_window_sums_shared<CWR, CWA>(M, S, p.pixels, n, r0, c0, dw, dh, win_rg, win_az,
+                                              smem, ws, c);
         else
-            tile_window_sums(M, S, p.pixels, n, r0, c0, dw, dh, p.win_rg, p.win_az, smem, ws, c);
+            tile_window_sums(M, S, p.pixels, n, r0, c0, dw, dh, win_rg, win_az, smem, ws, c);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const float co = ws[k].coherence();
@@ -254,7 +257,9 @@ int launch_esd_bands(const tops_ci16* mf, const tops_ci16* mb, const tops_cf32* 
     p.tiles_y = maxn > 0 ? (maxn + TW_ROWS - 1) / TW_ROWS : 1;
     p.partials = (double2*)d_partials;
     const dim3 grid(p.tiles_x * p.tiles_y * n_overlap);
-    if (tile_window_shared_ok(win_rg, win_az))
+    if (win_rg == 24 && win_az == 6)   // the reference's ESD window, unrolled
+        hipLaunchKernelGGL((esd_band_kernel<true, 24, 6>), grid, dim3(256), lds, st, p);
+    else if (tile_window_shared_ok(win_rg, win_az))
         hipLaunchKernelGGL(esd_band_kernel<true>, grid, dim3(256), lds, st, p);
     else
         hipLaunchKernelGGL(esd_band_kernel<false>, grid, dim3(256), lds, st, p);

@@ -78,9 +78,12 @@ __device__ __forceinline__ float4 window_product(tops_ci16 m, tops_cf32 v)
 // any of them; its position advances by 256 with no division in the loop.
 // A window inside the raster (most tiles) takes a loop without the bounds
 // tests.
+template <int CPR = 0, int CPC = 0>
 __device__ __forceinline__ void tile_products(const tops_ci16* M, const tops_cf32* S, int w, int h,
-                                     int gr0, int gc0, int PR, int PC, float4* smem)
+                                     int gr0, int gc0, int PR_, int PC_, float4* smem)
 {
+    // compile-time tile shape when the caller fixes the window
+    const int PR = CPR ? CPR : PR_, PC = CPC ? CPC : PC_;
     const int n = PR * PC;
     const int step_r = 256 / PC, step_c = 256 - step_r * PC;
     int rr = (int)threadIdx.x / PC, cc = (int)threadIdx.x - rr * PC;
@@ -223,14 +226,19 @@ __host__ __device__ inline size_t tile_window_launch_lds(int win_rg, int win_az)
 //     31 product reads for 8 windows instead of 8 x 24;
 //   azimuth: the thread's 4 output rows share the range sums of rows
 //     3 .. winAz-1 in the same way (winAz + 3 reads for 4 outputs).
-// Ends with a barrier, so smem may be reused right after.
+// Ends with a barrier, so smem may be reused right after.  CWR / CWA fix
+// the window at compile time (0: the run-time win_rg / win_az): its loops
+// then unroll completely.
+template <int CWR = 0, int CWA = 0>
 __device__ inline void tile_window_sums_shared(const tops_ci16* M, const tops_cf32* S, int w,
                                                int h, int r0, int c0, int dx, int dy,
-                                               int win_rg, int win_az, float4* smem,
+                                               int win_rg_, int win_az_, float4* smem,
                                                WindowSums ws[4], float2* center)
 {
+    const int win_rg = CWR ? CWR : win_rg_, win_az = CWA ? CWA : win_az_;
     const int PR = TW_ROWS + win_az - 1, PC = TW_COLS + win_rg - 1;
-    tile_products(M, S, w, h, r0 - dy, c0 - dx, PR, PC, smem);
+    tile_products<CWA ? TW_ROWS + CWA - 1 : 0, CWR ? TW_COLS + CWR - 1 : 0>(
+        M, S, w, h, r0 - dy, c0 - dx, PR, PC, smem);
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (center) {
@@ -260,6 +268,9 @@ __device__ inline void tile_window_sums_shared(const tops_ci16* M, const tops_cf
         const float4* q = smem + trow * PC + trun * 8;
         // the middle first: its reads do not overlap the suffix sums' registers
         float4 mid = make_float4(0.f, 0.f, 0.f, 0.f);
+        // (a fixed window's middle unrolled by 4: fully unrolled, its reads
+        // all issue at once and the ESD kernel spills at 4 waves per SIMD)
+#pragma unroll 4
         for (int i = 7; i < win_rg; i++) mid = add4(mid, q[i]);
         float4 sfx[7];
         sfx[6] = q[6];
