@@ -287,6 +287,15 @@ def test_plan_reference_window_6x24(tops, orc, dev, batch):
         s.synchronize()
         for (b, d), ec in zip(zip(batch["bursts"], batch["dev"]), eager):
             assert np.array_equal(dev.get(d["coh"], np.float32, (b["L"], b["P"])), ec), rep
+    # the same window's tiled coherence in 3 burst groups (second stream)
+    N.check(L.tops_plan_set_groups(plan, 3), "tops_plan_set_groups")
+    for d in batch["dev"]:
+        d["coh"].zero_()
+    torch.cuda.synchronize()
+    N.check(L.tops_plan_run(plan, sp))
+    s.synchronize()
+    for (b, d), ec in zip(zip(batch["bursts"], batch["dev"]), eager):
+        assert np.array_equal(dev.get(d["coh"], np.float32, (b["L"], b["P"])), ec), "groups"
     L.tops_plan_destroy(plan)
 
 
