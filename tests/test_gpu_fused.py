@@ -193,6 +193,27 @@ def test_fused_bursts_without_coherence(tops, orc, dev):
         tops.native.load().tops_ctx_destroy(w["ctx"])
 
 
+def test_batched_groups_with_bursts_without_coherence(tops, orc, dev):
+    """The batched schedule in burst groups where some groups' bursts have
+    no coherence item (their coherence launch is skipped): outputs equal the
+    oracle, and the coherence buffers of those bursts are left alone."""
+    N = tops.native
+    L = N.load()
+    shapes = [(60, 500), (61, 480), (59, 520), (62, 510), (58, 490)]
+    w = _make(tops, dev, shapes, 24, coh=[False, True, False, False, True])
+    try:
+        n = len(shapes)
+        plan = L.tops_plan_create(w["ctx"], w["items"], n, 3, 10, 32)
+        assert plan, N.last_error()
+        N.check(L.tops_plan_set_groups(plan, 4), "tops_plan_set_groups")
+        assert L.tops_plan_groups(plan) == 4
+        N.check(L.tops_plan_run(plan, None), "tops_plan_run")
+        _check_oracle(orc, dev, w, "groups, some without coherence")
+        L.tops_plan_destroy(plan)
+    finally:
+        L.tops_ctx_destroy(w["ctx"])
+
+
 @pytest.mark.parametrize("az_scale", [40.0, 400.0])
 def test_fused_azimuth_shear(tops, orc, dev, az_scale):
     """Strong azimuth shear along the range: many azimuth keys per row, so
