@@ -211,6 +211,9 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
     }
 
     const double azs = az_shift_of(p);
+    // a strip whose outputs are all interior skips the per-output region
+    // tests (wave-uniform)
+    const bool all_interior = block_interior(p, r0, r1, c0, min(c0 + WK_OUT - 1, p.m_pixels - 1));
     WinRow win[WK_WIN];
     int wbase = INT_MIN;   // absolute slave line held in win[0]
 
@@ -252,7 +255,7 @@ __device__ __forceinline__ void resample_strip(const ResampleParams& p, int sx, 
             ix = (int)trunc(xs);
             fx = (int)((xs - ix) * kInterval + 0.5);
         }
-        if (out_lane) reg = region_of(p, iy, ix);
+        if (out_lane) reg = all_interior ? 2 : region_of(p, iy, ix);
         const int base = ix - kHalf - ucol0;
         const bool servable = reg == 2 && base >= 0 && base <= 64 - kTaps;
         const int key = servable ? iy * kLutRows + fy : INT_MAX;

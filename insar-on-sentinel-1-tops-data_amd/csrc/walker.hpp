@@ -46,6 +46,63 @@ __device__ inline int region_of(const ResampleParams& p, int iy, int ix)
     return 2;
 }
 
+// range of c0 + c1 x + c2 y + c3 x^2 + c4 x y + c5 y^2 over the box
+// [x0, x1] x [y0, y1] (x0 <= x1, y0 <= y1), by interval arithmetic on each
+// term; lo > hi (an empty range) when an input is not finite
+__device__ inline void poly6_range(const double* c, double x0, double x1, double y0, double y1,
+                                   double& lo, double& hi)
+{
+    auto lin = [](double k, double a, double b, double& l, double& h) {
+        l = fmin(k * a, k * b);
+        h = fmax(k * a, k * b);
+    };
+    auto sq = [](double a, double b, double& l, double& h) {   // range of t^2, t in [a, b]
+        h = fmax(a * a, b * b);
+        l = (a <= 0.0 && b >= 0.0) ? 0.0 : fmin(a * a, b * b);
+    };
+    double l, h, tl, th;
+    lo = hi = c[0];
+    lin(c[1], x0, x1, l, h); lo += l; hi += h;
+    lin(c[2], y0, y1, l, h); lo += l; hi += h;
+    sq(x0, x1, tl, th); lin(c[3], tl, th, l, h); lo += l; hi += h;
+    const double p0 = x0 * y0, p1 = x0 * y1, p2 = x1 * y0, p3 = x1 * y1;
+    tl = fmin(fmin(p0, p1), fmin(p2, p3));
+    th = fmax(fmax(p0, p1), fmax(p2, p3));
+    lin(c[4], tl, th, l, h); lo += l; hi += h;
+    sq(y0, y1, tl, th); lin(c[5], tl, th, l, h); lo += l; hi += h;
+    if (!(lo <= hi) || !isfinite(lo) || !isfinite(hi)) {
+        lo = 1.0;
+        hi = 0.0;
+    }
+}
+
+// Whether every output of rows r0 .. r1-1, columns c0 .. c1 of the master
+// box is interior (region 2): the ranges of both position polynomials over
+// the block lie, with a margin, inside the slave box's interior band.  Then
+// the walker need not test the region per output.
+__device__ inline bool block_interior(const ResampleParams& p, int r0, int r1, int c0, int c1)
+{
+#pragma clang fp contract(off)
+    const double l0 = norm_warp((double)(p.mb[2] + r0), (double)p.mb[2], (double)p.mb[3]);
+    const double l1 = norm_warp((double)(p.mb[2] + r1 - 1), (double)p.mb[2], (double)p.mb[3]);
+    const double q0 = norm_warp((double)(p.mb[0] + c0), (double)p.mb[0], (double)p.mb[1]);
+    const double q1 = norm_warp((double)(p.mb[0] + c1), (double)p.mb[0], (double)p.mb[1]);
+    double ylo, yhi, xlo, xhi;
+    poly6_range(p.cpm_az, fmin(l0, l1), fmax(l0, l1), fmin(q0, q1), fmax(q0, q1), ylo, yhi);
+    poly6_range(p.cpm_rg, fmin(l0, l1), fmax(l0, l1), fmin(q0, q1), fmax(q0, q1), xlo, xhi);
+    if (!(ylo <= yhi) || !(xlo <= xhi)) return false;
+    const double azs = az_shift_of(p);
+    ylo += azs;
+    yhi += azs;
+    // interior: kHalf <= trunc(pos) - box start <= box end - kHalf - 2, so
+    // pos in [start + kHalf, end - kHalf - 1); a margin of 1e-3 px plus 1e-9
+    // of the magnitude covers the rounding of both evaluations
+    const double my = 1e-3 + 1e-9 * (fabs(ylo) + fabs(yhi));
+    const double mx = 1e-3 + 1e-9 * (fabs(xlo) + fabs(xhi));
+    return ylo >= p.sb[2] + kHalf + my && yhi < p.sb[3] - kHalf - 1 - my &&
+           xlo >= p.sb[0] + kHalf + mx && xhi < p.sb[1] - kHalf - 1 - mx;
+}
+
 __device__ inline double deramp_phase_direct(const ResampleParams& p, int ya, int xa)
 {
     double ta = (ya - p.first_line) * p.dt;
