@@ -677,8 +677,8 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
         }
     }
     // per-burst launches (0), fixed batched heights, the batched heights
-    // that fill 1, 2 and 3 rounds of the GPU's walker wave slots, and the
-    // fused launch's row bands (negative)
+    // that fill 1, 2, 3, 5 and 8 rounds of the GPU's walker wave slots, and
+    // the fused launch's row bands (negative)
     std::vector<int> cand = {0, 32, 64, 128, 192, 256, 320, 384, 512};
     if (p->fused_ok)
         for (int r : {64, 128, 192, 256, 384, 512, 768}) cand.push_back(-r);
@@ -688,7 +688,7 @@ int tops_plan_tune(tops_plan* p, void* stream, int* best_rows)
             ls.push_back(it.geom.master_box[3] - it.geom.master_box[2] + 1);
             ps.push_back(it.geom.master_box[1] - it.geom.master_box[0] + 1);
         }
-        for (int r = 1; r <= 3; r++) {
+        for (int r : {1, 2, 3, 5, 8}) {
             int h = std::max(16, resample_rows_for_rounds(ls.data(), ps.data(), (int)ls.size(), r));
             if (std::find(cand.begin(), cand.end(), h) == cand.end()) cand.push_back(h);
         }

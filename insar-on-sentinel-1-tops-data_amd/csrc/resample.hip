@@ -478,7 +478,14 @@ int launch_resample_rows(const ResampleParams& p, int row0, int rows, hipStream_
     if ((long long)p.s_lines * p.s_pixels * 4 > kMaxSlaveBytes)
         return set_error(TOPS_ERR_ARG, "resample: slave box of %d x %d samples over 2^31 bytes",
                          p.s_lines, p.s_pixels);
-    if (strip_rows <= 0) strip_rows = resample_rows_for_rounds(&rows, &p.m_pixels, 1, 1);
+    if (strip_rows <= 0) {
+        // strips for about 5 rounds of the GPU's wave slots: on one IW burst
+        // (1514 x 21000) 31-34-row strips took 0.303-0.305 ms against 0.331 ms
+        // for one round of 169-row strips (the slots finish together); a
+        // launch of few rows keeps one round rather than strips under 24 rows
+        strip_rows = resample_rows_for_rounds(&rows, &p.m_pixels, 1, 5);
+        if (strip_rows < 24) strip_rows = resample_rows_for_rounds(&rows, &p.m_pixels, 1, 1);
+    }
     dim3 grid((p.m_pixels + WK_OUT - 1) / WK_OUT, (rows + strip_rows - 1) / strip_rows);
     hipLaunchKernelGGL(resample_fast_kernel, grid, dim3(64), 0, s, p, strip_rows, row0,
                        row0 + rows);
