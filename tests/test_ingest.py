@@ -369,3 +369,78 @@ def test_tiff_errors(tops, tmp_path):
     assert not L.tops_tiff_open(str(tmp_path / "txt.tif").encode())
     assert not L.tops_tiff_create(str(tmp_path / "x.tif").encode(), 3, 4, 4)
     N.load().tops_clear_error()
+
+
+# ---- corrupted files -------------------------------------------------------------
+def _mutants(rng, data, n):
+    """n seeded corruptions of data: truncations, byte changes, deleted and
+    repeated ranges."""
+    out = []
+    for _ in range(n):
+        b = bytearray(data)
+        for _ in range(int(rng.integers(1, 4))):
+            if not b:
+                break
+            i = int(rng.integers(0, len(b)))
+            k = int(rng.integers(0, 4))
+            if k == 0:
+                del b[i:]
+            elif k == 1:
+                b[i] = int(rng.integers(0, 256))
+            elif k == 2:
+                del b[i:i + int(rng.integers(1, 64))]
+            else:
+                b[i:i] = b[i:i + int(rng.integers(1, 200))]
+        out.append(bytes(b))
+    return out
+
+
+def test_readers_reject_corrupted_files(tops, sw, tmp_path):
+    """Corrupted annotation, EOF and TIFF files (seeded truncations, changed
+    bytes, deleted and repeated ranges) are either read or refused with
+    TOPS_ERR_ARG and a message: no reader crashes, reads past its buffers or
+    leaves a half-filled result (every successful read is freed again).  The
+    same mutations at a larger count ran clean against a build whose host
+    code had AddressSanitizer."""
+    N = tops.native
+    L = N.load()
+    rng = np.random.default_rng(20260103)
+    p = tmp_path / "f.bin"
+    ann = S1.annotation_xml(sw, decorate=True).encode()
+    for b in _mutants(rng, ann, 150):
+        p.write_bytes(b)
+        a = N.Annotation()
+        rc = L.tops_annotation_read(str(p).encode(), C.byref(a))
+        assert rc in (0, N.TOPS_ERR_ARG)
+        if rc == 0:
+            L.tops_annotation_free(C.byref(a))
+        else:
+            assert N.last_error()[1]
+    eof, *_ = _eof(tmp_path)
+    for b in _mutants(rng, eof.read_bytes(), 150):
+        p.write_bytes(b)
+        o = N.EofOrbit()
+        rc = L.tops_eof_read(str(p).encode(), b"2026-01-03T12:00:03.512",
+                             b"2026-01-03T12:00:30.2", C.byref(o))
+        assert rc in (0, N.TOPS_ERR_ARG)
+        if rc == 0:
+            L.tops_eof_free(C.byref(o))
+    for gdt, layout in [(8, "le"), (10, "be"), (6, "big"), (8, "strips")]:
+        q = tmp_path / f"t{gdt}{layout}.tif"
+        S1.write_tiff(q, _raster(gdt, 9, 7), gdt, big_endian=layout == "be",
+                      bigtiff=layout == "big", rows_per_strip=2 if layout == "strips" else 1,
+                      strip_gap=3 if layout == "strips" else 0)
+        for b in _mutants(rng, q.read_bytes(), 60):
+            p.write_bytes(b)
+            t = L.tops_tiff_open(str(p).encode())
+            if not t:
+                assert N.last_error()[1]
+                continue
+            info = (C.c_int(), C.c_int(), C.c_int())
+            if L.tops_tiff_info(t, *[C.byref(v) for v in info]) == 0:
+                w, h, g = (v.value for v in info)
+                if 0 < w <= 4096 and 0 < h <= 4096:
+                    buf = np.zeros(w * h * 16, np.uint8)
+                    assert L.tops_tiff_read(t, 0, 0, h, w, g, buf.ctypes.data) in (0, N.TOPS_ERR_ARG)
+            L.tops_tiff_close(t)
+    L.tops_clear_error()
