@@ -1,7 +1,7 @@
 /*
  * tops_oracle.c -- FP64 CPU restatement of the reference TOPS burst path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, by
+ * TEST INFRASTRUCTURE ONLY.  Loaded by tests/, by
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
  * product library.  Each function cites the reference file:line it follows.
  */

@@ -2,7 +2,7 @@
  * tops_oracle.h -- FP64 CPU restatement of the TOPS burst path of
  * a787854/InSAR-on-Sentinel-1-TOPS-data.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  The
  * product (the HIP library in the package) never links or calls it.
  *
