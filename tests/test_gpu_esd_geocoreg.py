@@ -142,6 +142,39 @@ def test_esd_thin_and_narrow_bands(tops, orc, dev):
     assert err <= TOL_ESD and serr <= 0.25
 
 
+@pytest.mark.parametrize("sizes", [[16, 32], [100, 40], [20, 5, 21]])
+def test_esd_band_heights(tops, orc, dev, sizes):
+    """The reference's 6 x 24 window over bands that end on a 16-row tile
+    boundary ([16, 32]), inside a tile row ([100, 40]: 4 and 8 kept rows in
+    the last one) and mixed ([20, 5, 21]), 300 samples wide (a partial tile
+    column), with a known double-difference phase: shifts and band sums
+    against the oracle."""
+    N = tops.native
+    L = N.load()
+    P, sizes = 300, np.array(sizes, np.int32)
+    rng = np.random.default_rng(int(sizes.sum()))
+    n = int(sizes.sum())
+    sf = (rng.normal(size=(n, P)) + 1j * rng.normal(size=(n, P))) * 200
+    sb = (rng.normal(size=(n, P)) + 1j * rng.normal(size=(n, P))) * 200
+    noise = lambda: (rng.normal(size=(n, P)) + 1j * rng.normal(size=(n, P))) * 40
+    q = lambda z: np.stack([np.round(z.real), np.round(z.imag)], -1).astype(np.int16)
+    mf, mb = q(sf * np.exp(0.25j) + noise()), q(sb + noise())
+    sf, sb = sf.astype(np.complex64), sb.astype(np.complex64)
+    k = len(sizes)
+    ref, rsums = orc.esd(mf, mb, sf.astype(np.complex128), sb.astype(np.complex128), sizes, P, 6,
+                         24, 0.3, 5400.0, 2.055556e-3)
+    d = [dev.put(np.ascontiguousarray(a)) for a in (mf, mb, sf, sb)]
+    shifts, sums = np.zeros(k), np.zeros(2 * k)
+    N.check(L.tops_esd_shifts(d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr(),
+                              sizes.ctypes.data_as(C.POINTER(C.c_int)), k, P, 6, 24, 0.3, 5400.0,
+                              2.055556e-3, N.ptr(shifts), N.ptr(sums), None))
+    err = float(np.max(np.abs(shifts - ref)))
+    serr = float(np.max(np.abs(sums.reshape(k, 2) - np.asarray(rsums))))
+    print(f"  bands {sizes.tolist()} x {P}: |d shift| = {err:.2e} px, sums |d| = {serr:.2e}")
+    assert np.all(np.abs(np.asarray(rsums)).sum(axis=1) > 0)
+    assert err <= TOL_ESD and serr <= 0.25
+
+
 def test_esd_zero_products_take_the_sign_of_zero(tops, dev):
     """Invalid (all-zero) forward master lines, as at a real burst's edge:
     every sample passes the mask with a double difference of signed zeros,
