@@ -258,7 +258,7 @@ __device__ inline void tile_window_sums_shared(const tops_ci16* M, const tops_cf
     // width, so its 16 float4 accesses still fall in 16 different bank
     // groups.  (Unpadded, the 21 rows of the 6 x 24 window put two rows of a
     // phase in one bank group: 16.95 M conflict cycles per 8 x 100 x 21000
-    // call, 1.18 M padded.)
+    // call, 1.18 M padded; 2.36 M in the compile-time 6 x 24 instance.)
     const int PRP = (PR + 7) & ~7;
     const int task = threadIdx.x;
     const int trow = task % PRP, trun = task / PRP;
