@@ -184,9 +184,10 @@ __device__ __forceinline__ void fused_item(CParams* it, const tops_ci16* cm, flo
         }
 
         // ---- this lane's output position, region and azimuth key.  Kept
-        // across the barriers packed: ky = Iy * 2048 + fy, kx = Ix * 2048 +
-        // fx (regions 1 and 2 lie in the slave box: 0 <= Iy, Ix < 2^20);
-        // reg 3 marks an interior output the shared pass can serve
+        // across the barriers packed: ky = Iy * 2048 + fy, kxp = Ix * 2048 +
+        // fx (regions 1 and 2 lie in the slave box: 0 <= Iy, Ix < 2^20, the
+        // plan offering this schedule only for slave boxes inside that
+        // range); reg 3 marks an interior output the shared pass can serve
         int ky = 0, kxp = 0, reg = -1, key = INT_MAX, mraw = 0;
         if (rs) {
             int iy, fy, ix, fx;
@@ -213,8 +214,16 @@ __device__ __forceinline__ void fused_item(CParams* it, const tops_ci16* cm, flo
                         reinterpret_cast<const char*>(cm) + (size_t)t * p.m_pixels * 4 +
                         lane_off(c, 4)));
             }
-            ky = iy * kLutRows + fy;
-            kxp = ix * kLutRows + fx;
+            // packed only for outputs in the slave box.  An edge output
+            // (region 1) takes the sample at the truncated index alone; its
+            // fraction is dropped, since a position in (-1, 0) truncates to 0
+            // with a negative fraction, which would unpack to line or column
+            // -1.  Interior positions are at least 5 lines and columns into
+            // the box, so their fractions are in 0 .. 2047.
+            if (reg >= 1) {
+                ky = iy * kLutRows + (reg >= 2 ? fy : 0);
+                kxp = ix * kLutRows + (reg >= 2 ? fx : 0);
+            }
             const int base = ix - kHalf - ucol0;
             if (reg == 2 && base >= 0 && base <= FK_LANES - kTaps) {
                 key = ky;

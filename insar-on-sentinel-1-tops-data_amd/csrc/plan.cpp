@@ -174,6 +174,10 @@ int fill_params(tops_plan* p)
     partition(p);
     p->fused_ok = fused_window_fits((p->coh_win_rg - 1) / 2, (p->coh_win_az - 1) / 2,
                                     p->coh_win_rg, p->coh_win_az);
+    // the fused kernel packs a slave column with its LUT row into one int
+    // (Ix * 2048 + fx): slave columns below 2^20, like the lines
+    for (const tops_burst_item& it : p->items)
+        if (it.geom.slave_box[1] >= kMaxSlaveLine) p->fused_ok = false;
     if (p->fused_ok) {
         if (!ci.empty()) {
             TOPS_HIP(hipMalloc(&p->d_coh_all, sizeof(CohItem) * ci.size()));

@@ -371,3 +371,57 @@ def test_plan_shifted_slave_origin(tops, orc, dev):
         assert not L.tops_plan_create(ctx, item, 1, 3, 10, 32)
         L.tops_clear_error()
     L.tops_ctx_destroy(ctx)
+
+
+@pytest.mark.parametrize("strip_rows", [0, 64, -64])
+def test_plan_azimuth_shifts(tops, orc, dev, strip_rows):
+    """Per-burst azimuth shifts (the ESD pass-2 shift of DerampDemodResample_ESD,
+    Func.cpp:2702-2705) through a plan's per-burst, batched and fused
+    schedules: three bursts shifted by 0, +0.37 and -1.6 lines against the
+    oracle resampled with the same shift, and the coherence of each."""
+    N = tops.native
+    L = N.load()
+    shifts = (0.0, 0.37, -1.6)
+    bursts, keep = [], []
+    kaz = krg = None
+    for i, sh in enumerate(shifts):
+        lines, pixels = 200 + 17 * i, 400 + 33 * i
+        pair = tops.synth.make_pair(f"sh{i}", lines, pixels, seed=300 + i, dc_slope_total=20.0 * i)
+        m = pair.master
+        if kaz is None:
+            kaz, krg = N.lut_init(12, m.prf, m.azimuth_bandwidth, 2 * m.range_sampling_rate)
+        kt, tref, fdc, _ = pair.params()
+        ca, cr = pair.cpm(0)
+        bursts.append(dict(M=pair.burst(0, "master"), S=pair.burst(0, "slave"), kt=kt[0].copy(),
+                           tref=tref[0].copy(), fdc=fdc[0].copy(), ca=ca, cr=cr, L=lines,
+                           P=pixels, dt=m.az_time_interval, sh=sh))
+    ctx = L.tops_ctx_create(N.ptr(kaz, C.c_float), N.ptr(krg, C.c_float), 12)
+    items = (N.BurstItem * len(bursts))()
+    for i, b in enumerate(bursts):
+        box = [0, b["P"] - 1, 0, b["L"] - 1]
+        g = N.BurstGeom((C.c_int * 4)(*box), (C.c_int * 4)(*box), 0, b["P"], b["dt"], b["sh"],
+                        (C.c_double * 6)(*b["ca"]), (C.c_double * 6)(*b["cr"]))
+        d = dict(S=dev.put(b["S"]), M=dev.put(b["M"]), kt=dev.put(b["kt"]),
+                 tref=dev.put(b["tref"]), fdc=dev.put(b["fdc"]),
+                 out=dev.empty(b["L"] * b["P"] * 8), coh=dev.empty(b["L"] * b["P"] * 4), g=g)
+        keep.append(d)
+        items[i] = N.BurstItem(g, d["S"].data_ptr(), d["kt"].data_ptr(), d["tref"].data_ptr(),
+                               d["fdc"].data_ptr(), d["out"].data_ptr(), d["M"].data_ptr(),
+                               d["coh"].data_ptr())
+    plan = L.tops_plan_create(ctx, items, len(bursts), 3, 10, strip_rows)
+    assert plan, N.last_error()
+    N.check(L.tops_plan_run(plan, None))
+    for b, d in zip(bursts, keep):
+        box = [0, b["P"] - 1, 0, b["L"] - 1]
+        ref = orc.resample(b["S"], box, box, 0, b["dt"], b["sh"], b["ca"], b["cr"], kaz, krg,
+                           b["kt"], b["tref"], b["fdc"])
+        got = dev.get(d["out"], np.complex64, (b["L"], b["P"]))
+        e = rel_err(got, ref)
+        cref = orc.coherence(b["M"], ref, 4, 1, 10, 3)
+        cgot = dev.get(d["coh"], np.float32, (b["L"], b["P"]))
+        ec = float(np.max(np.abs(cgot - cref)))
+        print(f"  schedule {strip_rows}, shift {b['sh']:+.2f} lines: resample {e:.2e}, "
+              f"coherence {ec:.2e}")
+        assert e <= 1e-5 and ec <= 1e-4
+    L.tops_plan_destroy(plan)
+    L.tops_ctx_destroy(ctx)
