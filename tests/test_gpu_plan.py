@@ -377,11 +377,14 @@ def test_plan_shifted_slave_origin(tops, orc, dev):
 def test_plan_azimuth_shifts(tops, orc, dev, strip_rows):
     """Per-burst azimuth shifts (the ESD pass-2 shift of DerampDemodResample_ESD,
     Func.cpp:2702-2705) through a plan's per-burst, batched and fused
-    schedules: three bursts shifted by 0, +0.37 and -1.6 lines against the
-    oracle resampled with the same shift, and the coherence of each."""
+    schedules: three bursts shifted by 0, +0.37 and -1.6 lines, the last two
+    also moved by -1.6 and +1.6 pixels in range (positions in (-1, 0) at the
+    top and left edges, past the box at the right edge), against the oracle
+    resampled with the same offsets, and the coherence of each."""
     N = tops.native
     L = N.load()
     shifts = (0.0, 0.37, -1.6)
+    rg_moves = (0.0, -1.6, 1.6)
     bursts, keep = [], []
     kaz = krg = None
     for i, sh in enumerate(shifts):
@@ -392,9 +395,11 @@ def test_plan_azimuth_shifts(tops, orc, dev, strip_rows):
             kaz, krg = N.lut_init(12, m.prf, m.azimuth_bandwidth, 2 * m.range_sampling_rate)
         kt, tref, fdc, _ = pair.params()
         ca, cr = pair.cpm(0)
+        cr = cr.copy()
+        cr[0] += rg_moves[i]
         bursts.append(dict(M=pair.burst(0, "master"), S=pair.burst(0, "slave"), kt=kt[0].copy(),
                            tref=tref[0].copy(), fdc=fdc[0].copy(), ca=ca, cr=cr, L=lines,
-                           P=pixels, dt=m.az_time_interval, sh=sh))
+                           P=pixels, dt=m.az_time_interval, sh=sh, rg=rg_moves[i]))
     ctx = L.tops_ctx_create(N.ptr(kaz, C.c_float), N.ptr(krg, C.c_float), 12)
     items = (N.BurstItem * len(bursts))()
     for i, b in enumerate(bursts):
@@ -420,8 +425,8 @@ def test_plan_azimuth_shifts(tops, orc, dev, strip_rows):
         cref = orc.coherence(b["M"], ref, 4, 1, 10, 3)
         cgot = dev.get(d["coh"], np.float32, (b["L"], b["P"]))
         ec = float(np.max(np.abs(cgot - cref)))
-        print(f"  schedule {strip_rows}, shift {b['sh']:+.2f} lines: resample {e:.2e}, "
-              f"coherence {ec:.2e}")
+        print(f"  schedule {strip_rows}, shift {b['sh']:+.2f} lines, range {b['rg']:+.2f} px: "
+              f"resample {e:.2e}, coherence {ec:.2e}")
         assert e <= 1e-5 and ec <= 1e-4
     L.tops_plan_destroy(plan)
     L.tops_ctx_destroy(ctx)

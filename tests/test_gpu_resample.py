@@ -197,6 +197,32 @@ def test_coherence_small_burst(tops, orc, dev, c1, rs):
             assert e <= TOL_COH
 
 
+@pytest.mark.parametrize("waz,wrg,dx,dy", [(3, 10, 0, 0), (3, 10, 9, 2), (3, 10, 5, 1),
+                                            (6, 24, 0, 5), (4, 9, 1, 3), (1, 10, 9, 0)])
+def test_coherence_offset_windows(tops, orc, dev, c1, rs, waz, wrg, dx, dy):
+    """Windows that start dx columns and dy rows before the sample other than
+    the centred (win - 1) / 2 (CoherenceEst's dx, dy arguments,
+    EstCoh.cu:101-139, 430-437): the 3 x 10 window off its walking form's
+    offsets takes the tile kernel, the others as their size selects; fast
+    and FP64 modes against the oracle's window sums."""
+    L, P = c1["L"], c1["P"]
+    box = [0, P - 1, 0, L - 1]
+    got, dres = rs.run(c1["S"], box, box, 0, c1["dt"], 0.0, c1["ca"], c1["cr"], c1["kt"],
+                       c1["tref"], c1["fdc"], 0)
+    N = tops.native
+    Lb = N.load()
+    dm = dev.put(c1["M"])
+    ref = orc.coherence(c1["M"], got.astype(np.complex128), dx, dy, wrg, waz)
+    for mode in (0, 1):
+        dc = dev.empty(L * P * 4)
+        N.check(Lb.tops_coherence(dm.data_ptr(), dres.data_ptr(), P, L, dx, dy, wrg, waz,
+                                  dc.data_ptr(), mode, None), "tops_coherence")
+        coh = dev.get(dc, np.float32, (L, P))
+        e = float(np.max(np.abs(coh - ref)))
+        print(f"  coherence {waz}x{wrg} from ({dy}, {dx}) mode={mode}: max|dc| = {e:.3e}")
+        assert e <= TOL_COH
+
+
 def test_reference_entry_points(tops, orc, c1):
     """DerampDemodResample / _ESD + CoherenceEst through host buffers, as
     Func.cpp:2382 and 2701-2716 call them."""
