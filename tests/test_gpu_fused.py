@@ -181,6 +181,27 @@ def test_fused_small_and_odd_boxes(tops, orc, dev):
         tops.native.load().tops_ctx_destroy(w["ctx"])
 
 
+@pytest.mark.parametrize("rows,groups", [(0, 1), (16, 1), (64, 1), (16, 3)])
+def test_batched_small_and_odd_boxes(tops, orc, dev, rows, groups):
+    """The same small and odd bursts through the per-burst (0) and batched
+    schedules (strips of 16 and 64 rows; 16 rows also in 3 burst groups)."""
+    shapes = [(2, 2), (5, 3), (17, 250), (40, 235), (33, 236), (2, 300), (3, 469), (12, 700)]
+    w = _make(tops, dev, shapes, 21)
+    N = tops.native
+    L = N.load()
+    try:
+        plan = L.tops_plan_create(w["ctx"], w["items"], len(shapes), 3, 10, rows)
+        assert plan, N.last_error()
+        assert L.tops_plan_strip_rows(plan) == rows
+        if groups > 1:
+            N.check(L.tops_plan_set_groups(plan, groups))
+        N.check(L.tops_plan_run(plan, None), "tops_plan_run")
+        _check_oracle(orc, dev, w, f"schedule {rows} / {groups} groups")
+        L.tops_plan_destroy(plan)
+    finally:
+        L.tops_ctx_destroy(w["ctx"])
+
+
 def test_fused_bursts_without_coherence(tops, orc, dev):
     """Bursts whose item has no master: resampled, coherence left alone."""
     shapes = [(60, 500), (61, 480), (59, 520)]
