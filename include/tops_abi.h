@@ -10,7 +10,11 @@
  *     readable with tops_last_error() and are printed once to stderr (each
  *     Part 1 entry point clears the status on entry, so a check after the
  *     call reports that call);
- *   - per-call parameter blocks instead of module globals (reentrant);
+ *   - per-call parameter blocks instead of module globals (reentrant):
+ *     calls may run on several streams and from several host threads at
+ *     once, each thread with its own status; a context (tops_ctx) may be
+ *     shared, while a plan (tops_plan) and the fixed scratch of
+ *     tops_esd_shifts_noalloc serve one call at a time;
  *   - device buffers returned by DerampDemodResample_ESD/ResampleFirstBurst
  *     are dense [rows x cols] (pitch = cols) and are released by the
  *     CoherenceEst* call that consumes them, as in the reference.

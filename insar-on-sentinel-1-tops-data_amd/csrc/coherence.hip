@@ -190,15 +190,14 @@ int coherence_walk_count(int w, int h, int rows, int* nx, int* ny)
 // CU), at least 16 rows.
 static int coherence_walk_rows(int w, int h)
 {
-    static int slots = 0;
-    if (!slots) {
-        int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+    static std::atomic<int> cache[kMaxCachedDevices];
+    const int slots = per_device_cached(cache, 256 * 5, [](int dev) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             cus <= 0)
             cus = 256;
-        slots = cus * 5;
-    }
+        return cus * 5;
+    });
     const int nx = (w + CW_BAND - 1) / CW_BAND;
     const int ny = std::max(1, slots / nx);
     return std::max(16, (h + ny - 1) / ny);

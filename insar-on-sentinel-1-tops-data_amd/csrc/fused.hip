@@ -441,10 +441,8 @@ int fused_strip_count(int m_pixels) { return (m_pixels + FK_OWN - 1) / FK_OWN; }
 // queried once per device
 int fused_grid()
 {
-    static int grid[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (!grid[dev]) {
+    static std::atomic<int> grid[kMaxCachedDevices];
+    return per_device_cached(grid, 256, [](int dev) {
         int cus = 0, per = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             cus <= 0)
@@ -453,9 +451,8 @@ int fused_grid()
                 hipSuccess ||
             per <= 0)
             per = 1;
-        grid[dev] = cus * per;
-    }
-    return grid[dev];
+        return cus * per;
+    });
 }
 
 int launch_fused_batch(const ResampleParams* d_items, const CohItem* d_coh, const int* d_coh_of,

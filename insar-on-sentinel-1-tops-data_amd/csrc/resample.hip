@@ -432,17 +432,14 @@ int resample_strip_count(int m_lines, int m_pixels, int strip_rows, int* nx, int
 // SIMD (kWalkWavesPerSimd).  Queried once per device.
 int resample_wave_slots()
 {
-    static int slots[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256 * 4 * kWalkWavesPerSimd;
-    if (!slots[dev]) {
+    static std::atomic<int> slots[kMaxCachedDevices];
+    return per_device_cached(slots, 256 * 4 * kWalkWavesPerSimd, [](int dev) {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             cus <= 0)
             cus = 256;
-        slots[dev] = cus * 4 * kWalkWavesPerSimd;
-    }
-    return slots[dev];
+        return cus * 4 * kWalkWavesPerSimd;
+    });
 }
 
 // Strip height whose strips (each one wave) fill `rounds` rounds of the

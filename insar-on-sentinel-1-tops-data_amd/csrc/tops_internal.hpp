@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <atomic>
+
 #include "tops_abi.h"
 
 namespace tops {
@@ -24,6 +26,23 @@ constexpr int kHalf = kTaps / 2 - 1;              // c_Npoints2m1
 // slave lines a burst geometry may address (0 .. kMaxSlaveLine - 1): the
 // resample walker packs (line, LUT row) into one int as line * 2048 + row
 constexpr int kMaxSlaveLine = (1 << 20) - 64;
+
+// A value computed once per device from its attributes (occupancy, CU
+// count).  Concurrent first calls compute the same value, so relaxed loads
+// and stores suffice; devices past the cache take `fallback`.
+constexpr int kMaxCachedDevices = 64;
+template <class F>
+int per_device_cached(std::atomic<int>* cache, int fallback, F&& compute)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxCachedDevices) return fallback;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (!v) {
+        v = compute(dev);
+        cache[dev].store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
 // the fast kernels address the slave raster with 32-bit buffer offsets
 constexpr long long kMaxSlaveBytes = (1ll << 31) - 1;
 
