@@ -152,3 +152,44 @@ def test_esd_workspace_covers_empty_bands(tops):
         tiles_x = (2048 + 63) // 64
         rows = max(1, (max(ov) + 15) // 16)
         assert ws >= len(ov) * tiles_x * rows * 16 + len(ov) * 16, (ov, ws)
+
+
+def test_burst_geometry_rejected_before_device_work(tops):
+    """tops_resample_burst and tops_plan_create refuse a burst geometry the
+    kernels cannot address, before any device work (the context and buffer
+    pointers are only checked for null): empty boxes, slave columns outside
+    the per-range arrays, slave lines outside [0, 2^20 - 64), master lines
+    beyond +-(2^20 - 64), a slave box over 2^31 bytes of CInt16, an unknown
+    mode."""
+    N = tops.native
+    L = N.load()
+    fake = C.cast(C.create_string_buffer(64), C.c_void_p)   # never dereferenced
+    good = dict(mb=[0, 99, 0, 49], sb=[0, 99, 0, 49], spb=100)
+    cases = {
+        "empty master box": dict(mb=[10, 9, 0, 49]),
+        "empty slave box": dict(sb=[0, 99, 5, 4]),
+        "slave column past the per-range arrays": dict(sb=[0, 100, 0, 49]),
+        "negative slave column": dict(sb=[-1, 99, 0, 49]),
+        "negative slave line": dict(sb=[0, 99, -3, 49]),
+        "slave line past 2^20 - 64": dict(sb=[0, 99, 0, (1 << 20) - 64]),
+        "master line past 2^20 - 64": dict(mb=[0, 99, 0, (1 << 20) - 64]),
+        "slave box over 2^31 bytes": dict(sb=[0, 99999, 0, 5400], spb=100000),
+    }
+    for label, over in cases.items():
+        d = dict(good, **over)
+        g = N.BurstGeom((C.c_int * 4)(*d["mb"]), (C.c_int * 4)(*d["sb"]), 0, d["spb"], 2e-3, 0.0,
+                        (C.c_double * 6)(), (C.c_double * 6)())
+        L.tops_clear_error()
+        rc = L.tops_resample_burst(fake, C.byref(g), fake, fake, fake, fake, fake, 0, None)
+        assert rc == N.TOPS_ERR_ARG, label
+        msg = L.tops_last_error_message()
+        items = (N.BurstItem * 1)()
+        items[0] = N.BurstItem(g, fake, fake, fake, fake, fake, None, None)
+        assert not L.tops_plan_create(fake, items, 1, 3, 10, 64), label
+        assert L.tops_last_error() == N.TOPS_ERR_ARG, label
+        print(f"  {label}: {msg.decode()}")
+    g = N.BurstGeom((C.c_int * 4)(*good["mb"]), (C.c_int * 4)(*good["sb"]), 0, 100, 2e-3, 0.0,
+                    (C.c_double * 6)(), (C.c_double * 6)())
+    assert L.tops_resample_burst(fake, C.byref(g), fake, fake, fake, fake, fake, 7,
+                                 None) == N.TOPS_ERR_ARG
+    L.tops_clear_error()
