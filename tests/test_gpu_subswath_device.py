@@ -273,3 +273,25 @@ def test_subswath_device_wide_overlaps(tops, orc, dev, run, flow):
         er = rel_err(res[rows], ref)
         print(f"  wide overlaps: burst {b} resample {er:.2e}")
         assert er <= 1e-5
+
+
+def test_subswath_device_reference_coherence_window(tops, orc, dev, run, flow):
+    """The reference's own ESDAndCoh coherence window, 6 az x 24 rg
+    (Func.cpp:2583-2584), through the device flow: the shifts and the pass-2
+    bursts equal those of the 3 x 10 run (neither depends on the coherence
+    window), and each burst's coherence equals the oracle's window sums over
+    the flow's own resampled burst."""
+    m, lpb, spb, nb = run.m, run.lpb, run.spb, run.nb
+    sizes, _, _ = orc.overlap_sizes(m.burst_first_line_time, m.burst_last_line_time, 0, nb - 1,
+                                    m.az_time_interval)
+    res, coh, sh = _desc_run(tops, dev, run, flow["ca"], flow["cr"], nb, list(sizes),
+                             coh_win=(6, 24))
+    assert np.array_equal(sh[:nb - 1], flow["shifts"][:nb - 1])
+    assert np.array_equal(res, flow["res"])
+    ec = 0.0
+    for b in range(nb):
+        rows = slice(b * lpb, (b + 1) * lpb)
+        cref = orc.coherence(run.m_slc[rows], res[rows].astype(np.complex128), 11, 2, 24, 6)
+        ec = max(ec, float(np.max(np.abs(coh[rows] - cref))))
+    print(f"  6 x 24 coherence over {nb} bursts: max|d| = {ec:.2e}")
+    assert ec <= 1e-4
