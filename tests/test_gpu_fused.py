@@ -247,3 +247,52 @@ def test_fused_azimuth_shear(tops, orc, dev, az_scale):
         tops.native.load().tops_plan_destroy(plan)
     finally:
         tops.native.load().tops_ctx_destroy(w["ctx"])
+
+
+@pytest.mark.parametrize("rows", [0, 64, -48])
+def test_plans_on_absolute_burst_boxes(tops, orc, dev, rows):
+    """Bursts of a subswath at their own boxes (master and slave lines
+    b*lpb .. (b+1)*lpb - 1, first line b*lpb, as DerampAndResample passes
+    them, Func.cpp:2356-2371) through the per-burst, batched and fused
+    schedules: resample and coherence of every burst against the oracle."""
+    N = tops.native
+    L = N.load()
+    pair = tops.synth.subswath_iw1(n_bursts=3, lines=200, pixels=600)
+    m = pair.master
+    lpb, spb = m.lines_per_burst, m.samples_per_burst
+    kaz, krg = N.lut_init(12, m.prf, m.azimuth_bandwidth, 2 * m.range_sampling_rate)
+    kt, tref, fdc, _ = pair.params()
+    ctx = L.tops_ctx_create(N.ptr(kaz, C.c_float), N.ptr(krg, C.c_float), 12)
+    items = (N.BurstItem * m.n_bursts)()
+    keep, host = [], []
+    for b in range(m.n_bursts):
+        ca, cr = pair.cpm(b)
+        box = [0, spb - 1, b * lpb, (b + 1) * lpb - 1]
+        g = N.BurstGeom((C.c_int * 4)(*box), (C.c_int * 4)(*box), b * lpb, spb,
+                        m.az_time_interval, 0.0, (C.c_double * 6)(*ca), (C.c_double * 6)(*cr))
+        S, M = pair.burst(b, "slave"), pair.burst(b, "master")
+        d = dict(g=g, S=dev.put(S), M=dev.put(M), kt=dev.put(kt[b].copy()),
+                 tref=dev.put(tref[b].copy()), fdc=dev.put(fdc[b].copy()),
+                 out=dev.empty(lpb * spb * 8), coh=dev.empty(lpb * spb * 4))
+        items[b] = N.BurstItem(g, d["S"].data_ptr(), d["kt"].data_ptr(), d["tref"].data_ptr(),
+                               d["fdc"].data_ptr(), d["out"].data_ptr(), d["M"].data_ptr(),
+                               d["coh"].data_ptr())
+        keep.append(d)
+        host.append(dict(S=S, M=M, ca=ca, cr=cr, box=box, kt=kt[b], tref=tref[b], fdc=fdc[b]))
+    try:
+        plan = L.tops_plan_create(ctx, items, m.n_bursts, 3, 10, rows)
+        assert plan, N.last_error()
+        N.check(L.tops_plan_run(plan, None), "tops_plan_run")
+        for b, (d, h) in enumerate(zip(keep, host)):
+            ref = orc.resample(h["S"], h["box"], h["box"], b * lpb, m.az_time_interval, 0.0,
+                               h["ca"], h["cr"], kaz, krg, h["kt"], h["tref"], h["fdc"])
+            got = dev.get(d["out"], np.complex64, (lpb, spb))
+            e = rel_err(got, ref)
+            cref = orc.coherence(h["M"], got.astype(np.complex128), 4, 1, 10, 3)
+            ec = float(np.max(np.abs(dev.get(d["coh"], np.float32, (lpb, spb)) - cref)))
+            print(f"  schedule {rows}, burst {b} (lines {h['box'][2]}..{h['box'][3]}): "
+                  f"resample {e:.2e}, coherence {ec:.2e}")
+            assert e <= 1e-5 and ec <= 1e-4
+        L.tops_plan_destroy(plan)
+    finally:
+        L.tops_ctx_destroy(ctx)
