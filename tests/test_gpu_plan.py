@@ -430,3 +430,42 @@ def test_plan_azimuth_shifts(tops, orc, dev, strip_rows):
         assert e <= 1e-5 and ec <= 1e-4
     L.tops_plan_destroy(plan)
     L.tops_ctx_destroy(ctx)
+
+
+def test_plan_tune_cache(tops, dev, batch, tmp_path, monkeypatch):
+    """TOPS_TUNE_CACHE: a tuned plan appends "<key> strip coh groups"; a
+    plan over the same burst shapes and window takes that choice without
+    timing anything; other shapes get their own key and line; a cached
+    per-burst choice (strip_rows 0) is taken as it is."""
+    N = tops.native
+    L = N.load()
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("TOPS_TUNE_CACHE", str(cache))
+    items, n = batch["items"], len(batch["bursts"])
+    best = C.c_int(0)
+
+    def tuned(k):
+        plan = L.tops_plan_create(batch["ctx"], items, k, 3, 10, 0)
+        assert plan, N.last_error()
+        N.check(L.tops_plan_tune(plan, None, C.byref(best)), "tune")
+        return plan, (L.tops_plan_strip_rows(plan), L.tops_plan_coh_rows(plan),
+                      L.tops_plan_groups(plan)), L.tops_plan_tune_count(plan)
+
+    p1, choice1, timed1 = tuned(n)
+    lines = cache.read_text().split("\n")
+    assert timed1 > 0 and len([x for x in lines if x]) == 1
+    key, *vals = lines[0].split()
+    assert tuple(int(v) for v in vals) == choice1 and best.value == choice1[0]
+    p2, choice2, timed2 = tuned(n)
+    assert timed2 == 0 and choice2 == choice1
+    p3, choice3, timed3 = tuned(n - 1)
+    assert timed3 > 0
+    keys = [x.split()[0] for x in cache.read_text().split("\n") if x]
+    assert len(keys) == 2 and keys[0] != keys[1]
+    # a cached per-burst schedule is reused (not re-tuned)
+    cache.write_text(f"{key} 0 32 1\n")
+    p4, choice4, timed4 = tuned(n)
+    assert timed4 == 0 and choice4 == (0, 32, 1)
+    print(f"  tuned {choice1} ({timed1} runs timed), cached {choice2}, per-burst {choice4}")
+    for p in (p1, p2, p3, p4):
+        L.tops_plan_destroy(p)
