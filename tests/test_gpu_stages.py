@@ -99,3 +99,32 @@ def test_esd_and_coh_stage(orc, run, stages):
                   f"cols {np.unique(bad[:, 1])[:12].tolist()}; first {bad[0].tolist()}: "
                   f"got {got[tuple(bad[0])]} ref {ref[tuple(bad[0])]}")
         assert er <= 1e-5 and ec <= 1e-4
+
+
+def test_stages_on_a_burst_subrange(orc, run, stages):
+    """The three stages over bursts 1..2 only (config.txt burst0 / burstN), with
+    the reference's ESDAndCoh windows (6 x 24 ESD and coherence,
+    Func.cpp:2583-2584): offsets and pass 1 equal those of the whole-swath
+    run for the same bursts, the one shift equals its second shift (the ESD
+    of bursts 1 and 2 does not depend on burst 0), pass 2 of burst 2 and the
+    6 x 24 coherence of both bursts against the oracle."""
+    lpb, m = run.lpb, run.m
+    ca, cr = run.geometric_coreg(1, 2)
+    assert np.array_equal(ca, stages["ca"][1:3]) and np.array_equal(cr, stages["cr"][1:3])
+    res1 = run.deramp_and_resample(ca, cr, 1, 2)
+    assert np.array_equal(res1, stages["res1"][lpb:3 * lpb])
+    res2, coh, shifts = run.esd_and_coh(ca, cr, res1, 1, 2)
+    assert len(shifts) == 1 and abs(shifts[0] - stages["shifts"][1]) <= 1e-12
+    # burst 1 (the first of the range) keeps its pass-1 samples; burst 2 is
+    # resampled again with the shift and the master's line interval
+    assert np.array_equal(res2[:lpb], res1[:lpb])
+    ref = oracle_resample_burst(orc, run, 2, ca[1], cr[1], float(shifts[0]), m.az_time_interval)
+    er = rel_err(res2[lpb:], ref)
+    ec = 0.0
+    for k in range(2):
+        rows = slice(k * lpb, (k + 1) * lpb)
+        cref = orc.coherence(run.m_slc[(k + 1) * lpb:(k + 2) * lpb],
+                             res2[rows].astype(np.complex128), 11, 2, 24, 6)
+        ec = max(ec, float(np.max(np.abs(coh[rows] - cref))))
+    print(f"  bursts 1..2: shift {shifts[0]:.6f}, pass 2 {er:.2e}, coherence 6x24 {ec:.2e}")
+    assert er <= 1e-5 and ec <= 1e-4
