@@ -142,16 +142,16 @@ def test_esd_thin_and_narrow_bands(tops, orc, dev):
     assert err <= TOL_ESD and serr <= 0.25
 
 
-@pytest.mark.parametrize("sizes", [[16, 32], [100, 40], [20, 5, 21]])
-def test_esd_band_heights(tops, orc, dev, sizes):
+@pytest.mark.parametrize("sizes,P", [([16, 32], 300), ([100, 40], 300), ([20, 5, 21], 301)])
+def test_esd_band_heights(tops, orc, dev, sizes, P):
     """The reference's 6 x 24 window over bands that end on a 16-row tile
     boundary ([16, 32]), inside a tile row ([100, 40]: 4 and 8 kept rows in
-    the last one) and mixed ([20, 5, 21]), 300 samples wide (a partial tile
-    column), with a known double-difference phase: shifts and band sums
-    against the oracle."""
+    the last one) and mixed ([20, 5, 21]), 300 or 301 samples wide (a partial
+    tile column, an odd width), with a known double-difference phase: shifts
+    and band sums against the oracle."""
     N = tops.native
     L = N.load()
-    P, sizes = 300, np.array(sizes, np.int32)
+    sizes = np.array(sizes, np.int32)
     rng = np.random.default_rng(int(sizes.sum()))
     n = int(sizes.sum())
     sf = (rng.normal(size=(n, P)) + 1j * rng.normal(size=(n, P))) * 200
