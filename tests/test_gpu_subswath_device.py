@@ -295,3 +295,93 @@ def test_subswath_device_reference_coherence_window(tops, orc, dev, run, flow):
         ec = max(ec, float(np.max(np.abs(coh[rows] - cref))))
     print(f"  6 x 24 coherence over {nb} bursts: max|d| = {ec:.2e}")
     assert ec <= 1e-4
+
+
+def test_subswath_device_slave_burst_size(tops, orc, dev, run, flow):
+    """A slave whose bursts are 2 lines and 2 samples larger than the
+    master's (products of two acquisitions need not share linesPerBurst or
+    samplesPerBurst; the slave box of DerampAndResample is its own,
+    Func.cpp:2356-2371): every slave burst padded at its end, the offset
+    polynomials moved by the 2 lines per burst.  Pass 1 (slave line interval,
+    slave line base), the ESD shifts, pass 2 (master line interval and base,
+    Func.cpp:2702-2705) and the coherence against the oracle on the same
+    inputs."""
+    import ctypes as C
+    N = tops.native
+    L = N.load()
+    torch = dev.torch
+    m, s = run.m, run.s
+    nb, lpb, spb = run.nb, run.lpb, run.spb
+    lps, sps = lpb + 2, spb + 2
+    so = run.orbits["s"]
+    sp = np.zeros((nb * lps, sps, 2), np.int16)
+    for b in range(nb):
+        sp[b * lps:b * lps + lpb, :spb] = run.s_slc[b * lpb:(b + 1) * lpb]
+        sp[b * lps + lpb:(b + 1) * lps, :spb] = run.s_slc[(b + 1) * lpb - 1]   # repeated last line
+        sp[b * lps:(b + 1) * lps, spb:] = sp[b * lps:(b + 1) * lps, spb - 1:spb]
+    pad = lambda a: np.concatenate([a, np.repeat(a[:, -1:], 2, axis=1)], axis=1)
+    kt, tref, fdc = (np.ascontiguousarray(pad(np.asarray(so[k], np.float64)))
+                     for k in ("kt", "tref", "fdc"))
+    ca, cr = flow["ca"].copy(), flow["cr"].copy()
+    for b in range(nb):
+        ca[b][0] += 2 * b          # slave line base b * lps instead of b * lpb
+    sizes, _, _ = orc.overlap_sizes(m.burst_first_line_time, m.burst_last_line_time, 0, nb - 1,
+                                    m.az_time_interval)
+    put = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+    ov = np.ascontiguousarray(sizes, np.int32)
+    df = L.tops_spectral_sep(N.ptr(run.orbits["m"]["kt"]), m.n_bursts, spb, lpb,
+                             m.az_time_interval)
+    desc = N.SubswathDesc(nb, lpb, spb, lps, sps, m.az_time_interval, s.az_time_interval, df,
+                          N.ptr(ov, C.c_int), 6, 24, 0.3, 3, 10)
+    ws = L.tops_subswath_workspace_size(C.byref(desc))
+    assert ws > 0, N.last_error()
+    d_m, d_s = put(run.m_slc), put(sp)
+    d_kt, d_tref, d_fdc = put(kt), put(tref), put(fdc)
+    d_ws = torch.empty(ws, dtype=torch.uint8, device="cuda")
+    d_res = torch.empty(nb * lpb * spb * 8, dtype=torch.uint8, device="cuda")
+    d_coh = torch.empty(nb * lpb * spb * 4, dtype=torch.uint8, device="cuda")
+    d_sh = torch.zeros(nb - 1, dtype=torch.float64, device="cuda")
+    ctx = L.tops_ctx_create(N.ptr(run.kaz, C.c_float), N.ptr(run.krg, C.c_float), 12)
+    try:
+        N.check(L.tops_subswath_esd_coh(ctx, C.byref(desc), d_m.data_ptr(), d_s.data_ptr(),
+                                        d_kt.data_ptr(), d_tref.data_ptr(), d_fdc.data_ptr(),
+                                        N.ptr(ca), N.ptr(cr), d_res.data_ptr(), d_coh.data_ptr(),
+                                        d_sh.data_ptr(), d_ws.data_ptr(), ws, None),
+                "tops_subswath_esd_coh")
+        torch.cuda.synchronize()
+    finally:
+        L.tops_ctx_destroy(ctx)
+    res = d_res.cpu().numpy().view(np.complex64).reshape(nb * lpb, spb)
+    coh = d_coh.cpu().numpy().view(np.float32).reshape(nb * lpb, spb)
+    sh = d_sh.cpu().numpy()
+
+    def oracle(b, first_line, dt, shift):
+        mbox = [0, spb - 1, b * lpb, (b + 1) * lpb - 1]
+        sbox = [0, sps - 1, b * lps, (b + 1) * lps - 1]
+        return orc.resample(sp[b * lps:(b + 1) * lps], mbox, sbox, first_line, dt, shift, ca[b],
+                            cr[b], run.kaz, run.krg, kt[b], tref[b], fdc[b])
+
+    r1 = np.concatenate([oracle(b, b * lps, s.az_time_interval, 0.0) for b in range(nb)])
+    M = run.m_slc
+    mf = np.concatenate([M[(i + 1) * lpb - n:(i + 1) * lpb] for i, n in enumerate(sizes)])
+    mb = np.concatenate([M[(i + 1) * lpb:(i + 1) * lpb + n] for i, n in enumerate(sizes)])
+    sf = np.concatenate([r1[(i + 1) * lpb - n:(i + 1) * lpb] for i, n in enumerate(sizes)])
+    sb = np.concatenate([r1[(i + 1) * lpb:(i + 1) * lpb + n] for i, n in enumerate(sizes)])
+    ref_sh, _ = orc.esd(mf, mb, sf, sb, sizes, spb, 6, 24, 0.3, df, m.az_time_interval)
+    ref_sh = np.where(np.abs(ref_sh) > 100, 0.0, ref_sh)
+    es = float(np.max(np.abs(sh - ref_sh)))
+    er = rel_err(res[:lpb], r1[:lpb])
+    ec = 0.0
+    for b in range(nb):
+        rows = slice(b * lpb, (b + 1) * lpb)
+        if b > 0:
+            # with the shift the device applied (checked against the oracle's
+            # above): a 1e-9-line change of the shift can move an azimuth
+            # position across a LUT-row boundary, a step of the resample itself
+            er = max(er, rel_err(res[rows], oracle(b, b * lpb, m.az_time_interval,
+                                                   float(sh[b - 1]))))
+        cref = orc.coherence(M[rows], res[rows].astype(np.complex128), 4, 1, 10, 3)
+        ec = max(ec, float(np.max(np.abs(coh[rows] - cref))))
+    print(f"  slave bursts {lps} x {sps} (master {lpb} x {spb}): shifts |d| {es:.2e}, "
+          f"resample {er:.2e}, coherence {ec:.2e}")
+    assert es <= 1e-4 and er <= 1e-5 and ec <= 1e-4
