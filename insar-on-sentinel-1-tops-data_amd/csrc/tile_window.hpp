@@ -176,6 +176,9 @@ __device__ inline void tile_window_sums(const tops_ci16* M, const tops_cf32* S, 
         ws[k].a0 = a0; ws[k].a1 = a1; ws[k].a2 = a2; ws[k].a3 = a3;
         if (center) {
             float4 v = sP[(rr + dy) * PC + lane + dx];
+            // read as the whole 16-byte entry: 8-byte reads at this 16-byte
+            // lane stride put two lanes of each half-wave on one bank pair
+            asm volatile("" :: "v"(v.z), "v"(v.w));
             center[k] = make_float2(v.x, v.y);
         }
     }
@@ -245,6 +248,9 @@ __device__ inline void tile_window_sums_shared(const tops_ci16* M, const tops_cf
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             float4 v = smem[(wv * 4 + k + dy) * PC + lane + dx];
+            // the whole 16-byte entry, as in tile_window_sums (8-byte reads
+            // here were all of the ESD kernel's 2.4 M conflict cycles per call)
+            asm volatile("" :: "v"(v.z), "v"(v.w));
             center[k] = make_float2(v.x, v.y);
         }
     }
@@ -258,7 +264,7 @@ __device__ inline void tile_window_sums_shared(const tops_ci16* M, const tops_cf
     // width, so its 16 float4 accesses still fall in 16 different bank
     // groups.  (Unpadded, the 21 rows of the 6 x 24 window put two rows of a
     // phase in one bank group: 16.95 M conflict cycles per 8 x 100 x 21000
-    // call, 1.18 M padded; 2.36 M in the compile-time 6 x 24 instance.)
+    // call, 1.18 M padded.)
     const int PRP = (PR + 7) & ~7;
     const int task = threadIdx.x;
     const int trow = task % PRP, trun = task / PRP;
