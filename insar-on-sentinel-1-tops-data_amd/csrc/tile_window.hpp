@@ -86,19 +86,28 @@ __device__ __forceinline__ void tile_products(const tops_ci16* M, const tops_cf3
     const int PR = CPR ? CPR : PR_, PC = CPC ? CPC : PC_;
     const int n = PR * PC;
     const int step_r = 256 / PC, step_c = 256 - step_r * PC;
-    int rr = (int)threadIdx.x / PC, cc = (int)threadIdx.x - rr * PC;
-    if (gr0 >= 0 && gr0 + PR <= h && gc0 >= 0 && gc0 + PC <= w) {
-        const tops_ci16* Mt = M + (size_t)gr0 * w + gc0;
-        const tops_cf32* St = S + (size_t)gr0 * w + gc0;
+    // the thread index through an empty asm: the positions below are formed
+    // on each call instead of being hoisted out of a caller's loop (the ESD
+    // kernel's two bands), where they held about 32 VGPRs across it
+    int tid = (int)threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    tid &= 255;   // its range again (256 threads), for the bounds below
+    int rr = tid / PC, cc = tid - rr * PC;
+    // inside the raster, and its byte offsets from the window's first sample
+    // (< PR * w * 8 <= 64 * 2^23 * 8) fit 32 bits: one uniform base plus a
+    // 32-bit offset per load
+    if (gr0 >= 0 && gr0 + PR <= h && gc0 >= 0 && gc0 + PC <= w && w <= (1 << 23)) {
+        const char* Mt = reinterpret_cast<const char*>(M + (size_t)gr0 * w + gc0);
+        const char* St = reinterpret_cast<const char*>(S + (size_t)gr0 * w + gc0);
         for (int q0 = 0; q0 < n; q0 += 8 * 256) {
             tops_ci16 m[8];
             tops_cf32 v[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                if (q0 + k * 256 + (int)threadIdx.x < n) {
-                    const int g = rr * w + cc;   // < PR * w: int
-                    m[k] = Mt[g];
-                    v[k] = St[g];
+                if (q0 + k * 256 + tid < n) {
+                    const unsigned g = (unsigned)(rr * w + cc);   // < PR * w
+                    m[k] = *reinterpret_cast<const tops_ci16*>(Mt + g * 4u);
+                    v[k] = *reinterpret_cast<const tops_cf32*>(St + g * 8u);
                 }
                 rr += step_r;
                 cc += step_c;
@@ -106,7 +115,7 @@ __device__ __forceinline__ void tile_products(const tops_ci16* M, const tops_cf3
             }
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const int q = q0 + k * 256 + (int)threadIdx.x;
+                const int q = q0 + k * 256 + tid;
                 if (q < n) smem[q] = window_product(m[k], v[k]);
             }
         }
@@ -117,7 +126,7 @@ __device__ __forceinline__ void tile_products(const tops_ci16* M, const tops_cf3
         tops_cf32 v[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const int q = q0 + k * 256 + (int)threadIdx.x;
+            const int q = q0 + k * 256 + tid;
             const int gr = gr0 + rr, gc = gc0 + cc;
             m[k].re = 0; m[k].im = 0;
             v[k].x = 0.f; v[k].y = 0.f;
@@ -132,7 +141,7 @@ __device__ __forceinline__ void tile_products(const tops_ci16* M, const tops_cf3
         }
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            const int q = q0 + k * 256 + (int)threadIdx.x;
+            const int q = q0 + k * 256 + tid;
             if (q < n) smem[q] = window_product(m[k], v[k]);
         }
     }
