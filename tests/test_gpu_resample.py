@@ -256,3 +256,33 @@ def test_reference_entry_points(tops, orc, c1):
     Lb.DerampDemodResample(*args)
     assert Lb.tops_last_error() == N.TOPS_ERR_ARG
     Lb.tops_clear_error()
+
+
+def test_coherence_wide_raster(tops, orc, dev):
+    """A raster wider than 2^23 samples: the tile kernels address their
+    product loads as a 32-bit byte offset from the window's first sample up
+    to that width, and take their bounds-tested loads beyond it.  Output
+    slices (the start, the middle, across column 2^23, the end) against the
+    oracle on the same slices widened by 32 columns each side (the windows
+    reach at most 12 columns across, so the margins hold every term)."""
+    N = tops.native
+    Lb = N.load()
+    h, w = 20, (1 << 23) + 70
+    rng = np.random.default_rng(23)
+    M = rng.integers(-500, 500, (h, w, 2), dtype=np.int16)
+    S = rng.standard_normal((h, w, 2), dtype=np.float32)
+    dm, ds = dev.put(M), dev.put(S)
+    dc = dev.empty(h * w * 4)
+    for waz, wrg in [(6, 24), (5, 7)]:
+        dx, dy = (wrg - 1) // 2, (waz - 1) // 2
+        N.check(Lb.tops_coherence(dm.data_ptr(), ds.data_ptr(), w, h, dx, dy, wrg, waz,
+                                  dc.data_ptr(), 0, None), "tops_coherence")
+        coh = dev.get(dc, np.float32, (h, w))
+        for c0, c1 in [(0, 200), (w // 2, w // 2 + 200), ((1 << 23) - 100, (1 << 23) + 40),
+                       (w - 200, w)]:
+            a, b = max(0, c0 - 32), min(w, c1 + 32)
+            sub = S[:, a:b, 0].astype(np.float64) + 1j * S[:, a:b, 1]
+            ref = orc.coherence(M[:, a:b], sub, dx, dy, wrg, waz)
+            e = float(np.max(np.abs(coh[:, c0:c1] - ref[:, c0 - a:c1 - a])))
+            print(f"  {waz}x{wrg} columns {c0}..{c1 - 1} of {w}: max|dc| = {e:.3e}")
+            assert e <= TOL_COH
