@@ -91,7 +91,7 @@ def _surface(ca, cr):
 def bench_geocoreg(tops, torch, pair, L, N, steps, check):
     """DEM-assisted coregistration of the burst: FP64 Newton posts/s over the
     1-arcsec DEM window of the burst footprint (24-post margin), wall clock
-    per call (the call ends with the 34-sum readback and the 6x6 solve)."""
+    per call (the call ends with the 27-sum readback and the 6x6 solve)."""
     m, s = pair.master, pair.slave
     keep = []
 

@@ -324,23 +324,25 @@ int tops_geocoreg(const int32_t* d_dem, int lines, int pixels, double lat0, doub
     }
     (void)hipFree(d_part);
     if (rc) return rc;
+    // N[i][j] = sum a_i a_j over [1, L, P, L^2, LP, P^2]: the moment of the
+    // summed exponents of L and P (degree d = i + j, power of P p: d(d+1)/2 + p)
+    static const int eL[6] = {0, 1, 0, 2, 1, 0}, eP[6] = {0, 0, 1, 0, 1, 2};
     double N[36], ba[6], br[6];
-    int k = 0;
     for (int i = 0; i < 6; i++)
-        for (int j = i; j < 6; j++) {
-            N[i * 6 + j] = sum[k];
-            N[j * 6 + i] = sum[k];
-            k++;
+        for (int j = 0; j < 6; j++) {
+            const int d = eL[i] + eL[j] + eP[i] + eP[j], p = eP[i] + eP[j];
+            N[i * 6 + j] = sum[d * (d + 1) / 2 + p];
         }
     for (int i = 0; i < 6; i++) {
-        ba[i] = sum[21 + i];
-        br[i] = sum[27 + i];
+        ba[i] = sum[kGeoMoments + i];
+        br[i] = sum[kGeoMoments + 6 + i];
     }
-    if (n_used) *n_used = (int64_t)sum[33];
-    if (sum[33] < 6.0 || !chol_solve(N, 6, ba, br)) {
+    const double used = sum[0];   // the moment L^0 P^0
+    if (n_used) *n_used = (int64_t)used;
+    if (used < 6.0 || !chol_solve(N, 6, ba, br)) {
         for (int i = 0; i < 6; i++) cpm_az[i] = cpm_rg[i] = 0.0;
         return set_error(TOPS_ERR_NOT_PD,
-                         "geocoreg: normal matrix not positive definite (%.0f posts used)", sum[33]);
+                         "geocoreg: normal matrix not positive definite (%.0f posts used)", used);
     }
     for (int i = 0; i < 6; i++) {
         cpm_az[i] = ba[i];

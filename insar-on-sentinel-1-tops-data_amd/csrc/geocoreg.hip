@@ -26,9 +26,11 @@
 // the normal-equation sums.
 //
 // Here the design matrix is never formed: each thread projects one post
-// against both orbits and adds its weighted contribution to the 21 unique
-// normal-matrix entries, the 12 right-hand-side entries and a count; the
-// workgroup reduces them with wave shuffles and writes one FP64 partial row.
+// against both orbits and adds its weighted contribution to the 15 moments
+// L^i P^j (i + j <= 4) that the 21 unique normal-matrix entries are made of
+// (the first is the count of posts used) and to the 12 right-hand-side
+// entries; the workgroup reduces them with wave shuffles and writes one FP64
+// partial row.
 // A second kernel adds the partial rows in a fixed order; the 6x6 Cholesky
 // solve runs on the host.
 #include "tops_internal.hpp"
@@ -173,7 +175,11 @@ struct GeoArgs {
     double* lp;
 };
 
-__global__ void __launch_bounds__(GEO_THREADS) geocoreg_kernel(GeoArgs args)
+// 4 waves per SIMD: 128 VGPRs and 32 B per lane spilled, against 136 VGPRs
+// and 3 waves unbounded; 217 against 224 us on the 3.28 M posts of a
+// 1500 x 21000 burst's DEM window (the same Cpm)
+__global__ void __launch_bounds__(GEO_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+geocoreg_kernel(GeoArgs args)
 {
     typedef __attribute__((address_space(4))) const GeoArgs CGeoArgs;
     const GeoArgs& A = *(const GeoArgs*)(CGeoArgs*)__builtin_amdgcn_kernarg_segment_ptr();
@@ -221,19 +227,21 @@ __global__ void __launch_bounds__(GEO_THREADS) geocoreg_kernel(GeoArgs args)
             project(m, P, ml, mp);
             project(s, P, sl, sp);
             if (ml >= ymin && ml <= ymax && mp >= xmin && mp <= xmax) {
-                double L = norm_warp(ml, ymin, ymax), Pn = norm_warp(mp, xmin, xmax);
-                double a[6] = {1.0, L, Pn, L * L, L * Pn, Pn * Pn};
-                int k = 0;
+                const double L = norm_warp(ml, ymin, ymax), Pn = norm_warp(mp, xmin, xmax);
+                const double L2 = L * L, LP = L * Pn, P2 = Pn * Pn;
+                const double a[6] = {1.0, L, Pn, L2, LP, P2};
+                // the normal matrix's 21 entries a_i a_j are 15 moments
+                // L^i P^j (i + j <= 4): one sum each, the first the count
+                const double mom[kGeoMoments] = {1.0,     L,       Pn,      L2,      LP,
+                                                 P2,      L2 * L,  L2 * Pn, L * P2,  P2 * Pn,
+                                                 L2 * L2, L2 * LP, L2 * P2, LP * P2, P2 * P2};
 #pragma unroll
-                for (int i = 0; i < 6; i++)
-#pragma unroll
-                    for (int j = i; j < 6; j++) acc[k++] += a[i] * a[j];
+                for (int k = 0; k < kGeoMoments; k++) acc[k] += mom[k];
 #pragma unroll
                 for (int i = 0; i < 6; i++) {
-                    acc[21 + i] += a[i] * sl;
-                    acc[27 + i] += a[i] * sp;
+                    acc[kGeoMoments + i] += a[i] * sl;
+                    acc[kGeoMoments + 6 + i] += a[i] * sp;
                 }
-                acc[33] += 1.0;
             }
         }
         if (lp) {

@@ -211,6 +211,9 @@ int launch_geocoreg(const int32_t* dem, int lines, int pixels, double lat0,
                     int ymin, int ymax, double* d_partials, int nblocks,
                     double* d_lp, hipStream_t st);
 int geocoreg_blocks(int lines, int pixels);
-constexpr int kGeoSums = 34;   // 21 N + 6 rhs_az + 6 rhs_rg + count
+// geocoreg sums: the moments L^i P^j (i + j <= 4, by degree, then by the
+// power of P; the first is the post count), then 6 rhs_az and 6 rhs_rg
+constexpr int kGeoMoments = 15;
+constexpr int kGeoSums = kGeoMoments + 12;
 
 }  // namespace tops
