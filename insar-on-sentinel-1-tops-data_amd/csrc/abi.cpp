@@ -312,7 +312,7 @@ int tops_geocoreg(const int32_t* d_dem, int lines, int pixels, double lat0, doub
     hipStream_t st = (hipStream_t)stream;
     int nb = geocoreg_blocks(lines, pixels);
     double* d_part = nullptr;
-    TOPS_HIP(hipMalloc(&d_part, sizeof(double) * kGeoSums * (size_t)(nb + 1)));
+    TOPS_HIP(hipMalloc(&d_part, sizeof(double) * geocoreg_scratch_doubles(lines, pixels, nb)));
     rc = launch_geocoreg(d_dem, lines, pixels, lat0, lon0, dlat, dlon, nodata, om, os, xmin, xmax,
                          ymin, ymax, d_part, nb, d_lp, st);
     double sum[kGeoSums];

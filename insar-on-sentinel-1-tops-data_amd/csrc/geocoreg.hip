@@ -178,7 +178,21 @@ struct GeoArgs {
     int xmin, xmax, ymin, ymax;
     double* partials;
     double* lp;
+    const TileRef* refs;   // per tile, from geocoreg_tile_ref_kernel
 };
+
+// The FP64 reference point of every tile (its sines and cosines are about a
+// fifth of a wave's instructions per tile when every wave forms them)
+__global__ void __launch_bounds__(256) geocoreg_tile_ref_kernel(GeoRef g, int lines, int pixels,
+                                                                TileRef* refs)
+{
+    const int tiles_x = (pixels + GT_COLS - 1) / GT_COLS;
+    const long ntiles = (long)tiles_x * ((lines + GT_ROWS - 1) / GT_ROWS);
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= ntiles) return;
+    const int tr0 = (int)(i / tiles_x) * GT_ROWS, tc0 = (int)(i % tiles_x) * GT_COLS;
+    refs[i] = tile_reference(g, tr0 + GT_ROWS / 2, tc0 + GT_COLS / 2);
+}
 
 // 4 waves per SIMD: 128 VGPRs and 32 B per lane spilled, against 136 VGPRs
 // and 3 waves unbounded; 3% faster on the 3.28 M posts of a 1500 x 21000
@@ -219,7 +233,7 @@ geocoreg_kernel(GeoArgs args)
         int hgt = dem[t];
         double ml = -99999999.0, mp = -99999999.0, sl = -99999999.0, sp = -99999999.0;
         if ((float)hgt != (float)nodata) {
-            const TileRef ref = tile_reference(g, rc, cc);
+            const TileRef ref = A.refs[tile];   // tile_reference(g, rc, cc)
             // angles from the reference post: a few posts times the spacing
             const float dphi = -(float)(row - rc) * dlatf;
             const float dlam = (float)(col - cc) * dlonf;
@@ -298,6 +312,19 @@ int geocoreg_blocks(int lines, int pixels)
     return (int)b;
 }
 
+static long geocoreg_tiles(int lines, int pixels)
+{
+    return (long)((lines + GT_ROWS - 1) / GT_ROWS) * ((pixels + GT_COLS - 1) / GT_COLS);
+}
+
+// the partial rows, the final sums and the tiles' reference points
+size_t geocoreg_scratch_doubles(int lines, int pixels, int nblocks)
+{
+    static_assert(sizeof(TileRef) % sizeof(double) == 0, "table in doubles");
+    return (size_t)kGeoSums * (nblocks + 1) +
+           (size_t)geocoreg_tiles(lines, pixels) * (sizeof(TileRef) / sizeof(double));
+}
+
 int launch_geocoreg(const int32_t* dem, int lines, int pixels, double lat0, double lon0,
                     double dlat, double dlon, int nodata, const OrbitDev& m,
                     const OrbitDev& s, int xmin, int xmax, int ymin, int ymax,
@@ -324,6 +351,12 @@ int launch_geocoreg(const int32_t* dem, int lines, int pixels, double lat0, doub
     a.ymax = ymax;
     a.partials = d_partials;
     a.lp = d_lp;
+    TileRef* refs = reinterpret_cast<TileRef*>(d_partials + (size_t)kGeoSums * (nblocks + 1));
+    a.refs = refs;
+    const long ntiles = geocoreg_tiles(lines, pixels);
+    hipLaunchKernelGGL(geocoreg_tile_ref_kernel, dim3((unsigned)((ntiles + 255) / 256)), dim3(256),
+                       0, st, g, lines, pixels, refs);
+    TOPS_HIP(hipGetLastError());
     hipLaunchKernelGGL(geocoreg_kernel, dim3(nblocks), dim3(GEO_THREADS), 0, st, a);
     TOPS_HIP(hipGetLastError());
     hipLaunchKernelGGL(geocoreg_final_kernel, dim3(kGeoSums), dim3(256), 0, st, d_partials,

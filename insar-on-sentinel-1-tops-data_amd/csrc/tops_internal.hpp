@@ -211,6 +211,8 @@ int launch_geocoreg(const int32_t* dem, int lines, int pixels, double lat0,
                     int ymin, int ymax, double* d_partials, int nblocks,
                     double* d_lp, hipStream_t st);
 int geocoreg_blocks(int lines, int pixels);
+// doubles of device scratch launch_geocoreg takes at d_partials
+size_t geocoreg_scratch_doubles(int lines, int pixels, int nblocks);
 // geocoreg sums: the moments L^i P^j (i + j <= 4, by degree, then by the
 // power of P; the first is the post count), then 6 rhs_az and 6 rhs_rg
 constexpr int kGeoMoments = 15;
