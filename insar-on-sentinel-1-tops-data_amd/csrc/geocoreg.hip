@@ -40,10 +40,17 @@ namespace tops {
 constexpr int GEO_THREADS = 256;
 constexpr int GT_ROWS = 4, GT_COLS = 64;   // posts per tile (one FP64 reference each)
 
+// The orbit polynomials run over all 6 coefficient slots, unrolled: the slots
+// past ncoef hold zeros, and their terms add exact zeros (fma(0, x, c) = c,
+// fma(0, x, v) = v).
+constexpr int kOrbitSlots = 6;
+static_assert(sizeof(OrbitDev::cx) == kOrbitSlots * sizeof(double), "coefficient slots");
+
 __device__ inline void orbit_pos(const OrbitDev& o, double tn, double s[3])
 {
     s[0] = s[1] = s[2] = 0.0;
-    for (int i = o.ncoef - 1; i >= 0; --i) {
+#pragma unroll
+    for (int i = kOrbitSlots - 1; i >= 0; --i) {
         s[0] = fma(s[0], tn, o.cx[i]);
         s[1] = fma(s[1], tn, o.cy[i]);
         s[2] = fma(s[2], tn, o.cz[i]);
@@ -58,7 +65,8 @@ __device__ inline void orbit_vel10_acc100(const OrbitDev& o, double tn, double v
     v[0] = o.cx[1]; v[1] = o.cy[1]; v[2] = o.cz[1];
     a[0] = a[1] = a[2] = 0.0;
     double p1 = 1.0;   // tn^(i-2)
-    for (int i = 2; i < o.ncoef; ++i) {
+#pragma unroll
+    for (int i = 2; i < kOrbitSlots; ++i) {
         double pv = (double)i * (p1 * tn);             // i tn^(i-1)
         double pa = (double)((i - 1) * i) * p1;        // i(i-1) tn^(i-2)
         v[0] = fma(o.cx[i], pv, v[0]);
@@ -194,9 +202,8 @@ __global__ void __launch_bounds__(256) geocoreg_tile_ref_kernel(GeoRef g, int li
     refs[i] = tile_reference(g, tr0 + GT_ROWS / 2, tc0 + GT_COLS / 2);
 }
 
-// 4 waves per SIMD: 128 VGPRs and 32 B per lane spilled, against 136 VGPRs
-// and 3 waves unbounded; 3% faster on the 3.28 M posts of a 1500 x 21000
-// burst's DEM window (the same Cpm)
+// 4 waves per SIMD (127 VGPRs, no spill; unbounded the compiler takes 133
+// and 3 waves)
 __global__ void __launch_bounds__(GEO_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
 geocoreg_kernel(GeoArgs args)
 {
