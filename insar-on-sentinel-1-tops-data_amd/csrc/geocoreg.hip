@@ -79,15 +79,14 @@ __device__ inline void orbit_vel10_acc100(const OrbitDev& o, double tn, double v
     }
 }
 
-// Zero-Doppler Newton solve (Geocoreg_cuda.cu:272-294) and the line / pixel
-// of the root (302-307).  The normalised time (t - t_mid) / 10 is formed as a
-// product with 0.1 (within an ulp), and with v = V / 10, a = A / 100 the
-// step -(v.d) / (a.d - v.v) is -10 (V.d) / (A.d - V.V): one FP64 division
-// per step instead of eight.  The root is the same to far below the 1e-10 s
-// stopping step.
-__device__ inline void project(const OrbitDev& o, const double P[3], double& line, double& pixel)
+// Zero-Doppler Newton solve (Geocoreg_cuda.cu:272-294) from t: at most 11
+// steps, stopping at a step below 1e-10 s.  The normalised time
+// (t - t_mid) / 10 is formed as a product with 0.1 (within an ulp), and with
+// v = V / 10, a = A / 100 the step -(v.d) / (a.d - v.v) is
+// -10 (V.d) / (A.d - V.V): one FP64 division per step instead of eight.
+__device__ inline double zero_doppler_time(const OrbitDev& o, const double P[3], double t)
 {
-    double t = o.az_init, s[3], v[3], a[3];
+    double s[3], v[3], a[3];
     for (int it = 0; it <= 10; it++) {
         double tn = (t - o.mid_time) * 0.1;
         orbit_pos(o, tn, s);
@@ -98,6 +97,14 @@ __device__ inline void project(const OrbitDev& o, const double P[3], double& lin
         t += sol;
         if (fabs(sol) < 1e-10) break;
     }
+    return t;
+}
+
+// line and pixel of P at its zero-Doppler time t (Geocoreg_cuda.cu:302-307)
+__device__ inline void line_pixel(const OrbitDev& o, const double P[3], double t, double& line,
+                                  double& pixel)
+{
+    double s[3];
     double tn = (t - o.mid_time) * 0.1;
     orbit_pos(o, tn, s);
     line = o.line_offset + (t - o.first_line_time) / o.dt;
@@ -125,6 +132,11 @@ struct TileRef {
     double pr[3];   // ECEF of the reference post
     double nv[3];   // the ellipsoid normal there (cos lat cos lon, cos lat sin lon, sin lat)
     float sr, cr, slr, clr, nr, inv_w2;
+    // zero-Doppler times of the reference post on the master and slave
+    // orbits (solved from the orbits' own starting times): the starting
+    // times of the tile's posts, a few posts away, so their solves take
+    // fewer steps to the same roots
+    double t0[2];
 };
 
 __device__ inline TileRef tile_reference(const GeoRef& g, int row, int col)
@@ -191,15 +203,19 @@ struct GeoArgs {
 
 // The FP64 reference point of every tile (its sines and cosines are about a
 // fifth of a wave's instructions per tile when every wave forms them)
-__global__ void __launch_bounds__(256) geocoreg_tile_ref_kernel(GeoRef g, int lines, int pixels,
-                                                                TileRef* refs)
+__global__ void __launch_bounds__(256) geocoreg_tile_ref_kernel(GeoRef g, OrbitDev m,
+                                                                OrbitDev s, int lines,
+                                                                int pixels, TileRef* refs)
 {
     const int tiles_x = (pixels + GT_COLS - 1) / GT_COLS;
     const long ntiles = (long)tiles_x * ((lines + GT_ROWS - 1) / GT_ROWS);
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= ntiles) return;
     const int tr0 = (int)(i / tiles_x) * GT_ROWS, tc0 = (int)(i % tiles_x) * GT_COLS;
-    refs[i] = tile_reference(g, tr0 + GT_ROWS / 2, tc0 + GT_COLS / 2);
+    TileRef r = tile_reference(g, tr0 + GT_ROWS / 2, tc0 + GT_COLS / 2);
+    r.t0[0] = zero_doppler_time(m, r.pr, m.az_init);
+    r.t0[1] = zero_doppler_time(s, r.pr, s.az_init);
+    refs[i] = r;
 }
 
 // 4 waves per SIMD (127 VGPRs, no spill; unbounded the compiler takes 133
@@ -250,8 +266,8 @@ geocoreg_kernel(GeoArgs args)
             double P[3] = {fma(hd, ref.nv[0], ref.pr[0]) + (double)d[0],
                            fma(hd, ref.nv[1], ref.pr[1]) + (double)d[1],
                            fma(hd, ref.nv[2], ref.pr[2]) + (double)d[2]};
-            project(m, P, ml, mp);
-            project(s, P, sl, sp);
+            line_pixel(m, P, zero_doppler_time(m, P, ref.t0[0]), ml, mp);
+            line_pixel(s, P, zero_doppler_time(s, P, ref.t0[1]), sl, sp);
             if (ml >= ymin && ml <= ymax && mp >= xmin && mp <= xmax) {
                 const double L = norm_warp(ml, ymin, ymax), Pn = norm_warp(mp, xmin, xmax);
                 const double L2 = L * L, LP = L * Pn, P2 = Pn * Pn;
@@ -362,7 +378,7 @@ int launch_geocoreg(const int32_t* dem, int lines, int pixels, double lat0, doub
     a.refs = refs;
     const long ntiles = geocoreg_tiles(lines, pixels);
     hipLaunchKernelGGL(geocoreg_tile_ref_kernel, dim3((unsigned)((ntiles + 255) / 256)), dim3(256),
-                       0, st, g, lines, pixels, refs);
+                       0, st, g, m, s, lines, pixels, refs);
     TOPS_HIP(hipGetLastError());
     hipLaunchKernelGGL(geocoreg_kernel, dim3(nblocks), dim3(GEO_THREADS), 0, st, a);
     TOPS_HIP(hipGetLastError());
