@@ -215,6 +215,10 @@ __global__ void __launch_bounds__(256) geocoreg_tile_ref_kernel(GeoRef g, OrbitD
     TileRef r = tile_reference(g, tr0 + GT_ROWS / 2, tc0 + GT_COLS / 2);
     r.t0[0] = zero_doppler_time(m, r.pr, m.az_init);
     r.t0[1] = zero_doppler_time(s, r.pr, s.az_init);
+    // a solve that did not give a finite time leaves the posts the orbit's
+    // own starting time
+    if (!isfinite(r.t0[0])) r.t0[0] = m.az_init;
+    if (!isfinite(r.t0[1])) r.t0[1] = s.az_init;
     refs[i] = r;
 }
 
